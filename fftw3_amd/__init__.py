@@ -1,0 +1,341 @@
+"""Python binding of libfftw3_amd.so (the MI355X FFT engine) via ctypes.
+
+The product is the C-ABI library; this module mirrors its fftw3.h entry
+points (same names, argument meaning, NULL-plan error behaviour) so tests and
+the benchmark can drive it from numpy.  Arrays passed to plans are device
+pointers (ints); DeviceBuffer moves numpy data to and from device memory.
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfftw3_amd.so")
+
+FFTW_FORWARD = -1
+FFTW_BACKWARD = 1
+FFTW_MEASURE = 0
+FFTW_DESTROY_INPUT = 1 << 0
+FFTW_UNALIGNED = 1 << 1
+FFTW_CONSERVE_MEMORY = 1 << 2
+FFTW_EXHAUSTIVE = 1 << 3
+FFTW_PRESERVE_INPUT = 1 << 4
+FFTW_PATIENT = 1 << 5
+FFTW_ESTIMATE = 1 << 6
+FFTW_WISDOM_ONLY = 1 << 21
+
+_lib = None
+
+
+class IODim(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("is_", ctypes.c_int), ("os", ctypes.c_int)]
+
+
+class IODim64(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_ssize_t), ("is_", ctypes.c_ssize_t), ("os", ctypes.c_ssize_t)]
+
+
+def build(force=False):
+    """Generate the kernel catalog and compile the library (make)."""
+    subprocess.check_call(["python3", os.path.join(HERE, "gen_kernels.py")])
+    cmd = ["make", "-C", HERE, "-j%d" % max(1, min(16, os.cpu_count() or 1))]
+    if force:
+        cmd.append("-B")
+    subprocess.check_call(cmd)
+
+
+def lib():
+    """Load libfftw3_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libfftw3_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, u = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint
+    ipp = ctypes.POINTER(ctypes.c_int)
+    for P in ("fftw_", "fftwf_"):
+        def f(name, res, args):
+            fn = getattr(L, P + name)
+            fn.restype = res
+            fn.argtypes = args
+        f("plan_many_dft", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, ip, u])
+        f("plan_dft", vp, [ip, ipp, vp, vp, ip, u])
+        f("plan_dft_1d", vp, [ip, vp, vp, ip, u])
+        f("plan_dft_2d", vp, [ip, ip, vp, vp, ip, u])
+        f("plan_dft_3d", vp, [ip, ip, ip, vp, vp, ip, u])
+        f("plan_guru_dft", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, ip, u])
+        f("plan_guru64_dft", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, ip, u])
+        f("plan_many_dft_r2c", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, u])
+        f("plan_many_dft_c2r", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, u])
+        f("plan_dft_r2c", vp, [ip, ipp, vp, vp, u])
+        f("plan_dft_c2r", vp, [ip, ipp, vp, vp, u])
+        f("plan_dft_r2c_1d", vp, [ip, vp, vp, u])
+        f("plan_dft_c2r_1d", vp, [ip, vp, vp, u])
+        f("plan_dft_r2c_2d", vp, [ip, ip, vp, vp, u])
+        f("plan_dft_c2r_2d", vp, [ip, ip, vp, vp, u])
+        f("plan_dft_r2c_3d", vp, [ip, ip, ip, vp, vp, u])
+        f("plan_dft_c2r_3d", vp, [ip, ip, ip, vp, vp, u])
+        f("plan_guru_dft_r2c", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, u])
+        f("plan_guru_dft_c2r", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, u])
+        f("plan_guru64_dft_r2c", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, u])
+        f("plan_guru64_dft_c2r", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, u])
+        f("plan_guru_split_dft", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, vp, vp, u])
+        f("plan_many_r2r", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, vp, u])
+        f("execute", None, [vp])
+        f("execute_dft", None, [vp, vp, vp])
+        f("execute_dft_r2c", None, [vp, vp, vp])
+        f("execute_dft_c2r", None, [vp, vp, vp])
+        f("destroy_plan", None, [vp])
+        f("copy_plan", vp, [vp])
+        f("cleanup", None, [])
+        f("forget_wisdom", None, [])
+        f("set_timelimit", None, [ctypes.c_double])
+        f("export_wisdom_to_string", ctypes.c_void_p, [])
+        f("import_wisdom_from_string", ip, [ctypes.c_char_p])
+        f("sprint_plan", ctypes.c_void_p, [vp])
+        f("malloc", vp, [ctypes.c_size_t])
+        f("free", None, [vp])
+        f("flops", None, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_double)])
+        f("estimate_cost", ctypes.c_double, [vp])
+        f("cost", ctypes.c_double, [vp])
+        f("alignment_of", ip, [vp])
+        f("init_threads", ip, [])
+        f("plan_with_nthreads", None, [ip])
+        f("planner_nthreads", ip, [])
+    L.fftw_amd_init.restype = ctypes.c_int
+    L.fftw_amd_last_error.restype = ctypes.c_char_p
+    L.fftw_amd_set_stream.argtypes = [vp]
+    L.fftw_amd_get_stream.restype = vp
+    L.fftw_amd_plan_set_stream.argtypes = [vp, vp]
+    L.fftw_amd_set_async.argtypes = [ip]
+    for n in ("fftw_amd_memcpy_htod", "fftw_amd_memcpy_dtoh", "fftw_amd_memcpy_dtod"):
+        getattr(L, n).argtypes = [vp, vp, ctypes.c_size_t]
+        getattr(L, n).restype = ctypes.c_int
+    L.fftw_amd_memset.argtypes = [vp, ip, ctypes.c_size_t]
+    L.fftw_amd_synchronize.restype = ctypes.c_int
+    L.fftw_amd_mem_free.restype = ctypes.c_size_t
+    for n in ("fftw_amd_fill_uniform", "fftw_amd_fill_gauss"):
+        getattr(L, n).argtypes = [vp, ip, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64]
+        getattr(L, n).restype = ctypes.c_int
+    L.fftw_amd_plan_npasses.argtypes = [vp]
+    L.fftw_amd_plan_pass_info.argtypes = [vp, ip, ctypes.c_char_p, ip]
+    L.fftw_amd_plan_bytes.argtypes = [vp]
+    L.fftw_amd_plan_bytes.restype = ctypes.c_double
+    L.fftw_amd_plan_pass_bytes.argtypes = [vp, ip]
+    L.fftw_amd_plan_pass_bytes.restype = ctypes.c_double
+    L.fftw_amd_plan_time.argtypes = [vp, vp, vp, ip, ctypes.POINTER(ctypes.c_double)]
+    L.fftw_amd_plan_time.restype = ctypes.c_double
+    L.fftw_amd_event_create.restype = vp
+    L.fftw_amd_event_record.argtypes = [vp, vp]
+    L.fftw_amd_event_elapsed.argtypes = [vp, vp]
+    L.fftw_amd_event_elapsed.restype = ctypes.c_double
+    L.fftw_amd_event_destroy.argtypes = [vp]
+    libc = ctypes.CDLL(None)
+    libc.free.argtypes = [vp]
+    L._libc = libc
+    _lib = L
+    return L
+
+
+def prefix(prec):
+    return "fftw_" if prec == "double" else "fftwf_"
+
+
+def fn(prec, name):
+    return getattr(lib(), prefix(prec) + name)
+
+
+def last_error():
+    return lib().fftw_amd_last_error().decode()
+
+
+def init():
+    r = lib().fftw_amd_init()
+    if r != 0:
+        raise RuntimeError("fftw_amd_init failed: " + last_error())
+
+
+class DeviceBuffer:
+    """Device allocation from fftw_malloc (device memory)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().fftw_malloc(max(1, self.nbytes))
+        if not self.ptr:
+            raise MemoryError("fftw_malloc(%d) failed" % self.nbytes)
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        r = lib().fftw_amd_memcpy_htod(self.ptr + offset, a.ctypes.data, a.nbytes)
+        if r:
+            raise RuntimeError("htod copy failed (%d)" % r)
+
+    def download(self, dtype, count=None, offset=0):
+        dt = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dt.itemsize
+        out = np.empty(count, dtype=dt)
+        r = lib().fftw_amd_memcpy_dtoh(out.ctypes.data, self.ptr + offset, out.nbytes)
+        if r:
+            raise RuntimeError("dtoh copy failed (%d)" % r)
+        return out
+
+    def fill_uniform(self, prec, count, seed, stream=0):
+        r = lib().fftw_amd_fill_uniform(self.ptr, 0 if prec == "double" else 1, count, seed, stream)
+        if r:
+            raise RuntimeError("fill failed (%d)" % r)
+
+    def fill_gauss(self, prec, count, seed, stream=0):
+        r = lib().fftw_amd_fill_gauss(self.ptr, 0 if prec == "double" else 1, count, seed, stream)
+        if r:
+            raise RuntimeError("fill failed (%d)" % r)
+
+    def free(self):
+        if self.ptr:
+            lib().fftw_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Owns an fftw_plan / fftwf_plan handle."""
+
+    def __init__(self, handle, prec):
+        if not handle:
+            raise RuntimeError("planner returned NULL: " + last_error())
+        self.h = handle
+        self.prec = prec
+
+    def execute(self):
+        fn(self.prec, "execute")(self.h)
+
+    def execute_dft(self, i, o):
+        fn(self.prec, "execute_dft")(self.h, i, o)
+
+    def execute_r2c(self, i, o):
+        fn(self.prec, "execute_dft_r2c")(self.h, i, o)
+
+    def execute_c2r(self, i, o):
+        fn(self.prec, "execute_dft_c2r")(self.h, i, o)
+
+    def npasses(self):
+        return lib().fftw_amd_plan_npasses(self.h)
+
+    def pass_info(self):
+        out = []
+        for i in range(self.npasses()):
+            buf = ctypes.create_string_buffer(512)
+            lib().fftw_amd_plan_pass_info(self.h, i, buf, 512)
+            out.append(buf.value.decode())
+        return out
+
+    def bytes(self):
+        return lib().fftw_amd_plan_bytes(self.h)
+
+    def pass_bytes(self):
+        return [lib().fftw_amd_plan_pass_bytes(self.h, i) for i in range(self.npasses())]
+
+    def time(self, i, o, iters=10):
+        n = self.npasses()
+        per = (ctypes.c_double * max(1, n))()
+        tot = lib().fftw_amd_plan_time(self.h, i, o, iters, per)
+        if tot < 0:
+            raise RuntimeError("plan timing failed: " + last_error())
+        return tot, [per[k] for k in range(n)]
+
+    def sprint(self):
+        p = fn(self.prec, "sprint_plan")(self.h)
+        s = ctypes.cast(p, ctypes.c_char_p).value.decode()
+        lib()._libc.free(p)
+        return s
+
+    def destroy(self):
+        if self.h:
+            fn(self.prec, "destroy_plan")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _iarr(v):
+    if v is None:
+        return None
+    return (ctypes.c_int * len(v))(*v)
+
+
+def plan_many_dft(prec, n, howmany, inp, inembed, istride, idist, out, onembed, ostride, odist,
+                  sign, flags=FFTW_ESTIMATE):
+    h = fn(prec, "plan_many_dft")(len(n), _iarr(n), howmany, inp, _iarr(inembed), istride, idist,
+                                  out, _iarr(onembed), ostride, odist, sign, flags)
+    return Plan(h, prec)
+
+
+def plan_many_dft_r2c(prec, n, howmany, inp, inembed, istride, idist, out, onembed, ostride, odist,
+                      flags=FFTW_ESTIMATE):
+    h = fn(prec, "plan_many_dft_r2c")(len(n), _iarr(n), howmany, inp, _iarr(inembed), istride, idist,
+                                      out, _iarr(onembed), ostride, odist, flags)
+    return Plan(h, prec)
+
+
+def plan_many_dft_c2r(prec, n, howmany, inp, inembed, istride, idist, out, onembed, ostride, odist,
+                      flags=FFTW_ESTIMATE):
+    h = fn(prec, "plan_many_dft_c2r")(len(n), _iarr(n), howmany, inp, _iarr(inembed), istride, idist,
+                                      out, _iarr(onembed), ostride, odist, flags)
+    return Plan(h, prec)
+
+
+def _dims(dims, wide):
+    T = IODim64 if wide else IODim
+    arr = (T * max(1, len(dims)))()
+    for i, (n, s_in, s_out) in enumerate(dims):
+        arr[i].n, arr[i].is_, arr[i].os = n, s_in, s_out
+    return arr
+
+
+def plan_guru_dft(prec, dims, hdims, inp, out, sign, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_dft" if wide else "plan_guru_dft"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, out, sign, flags)
+    return Plan(h, prec)
+
+
+def plan_guru_dft_r2c(prec, dims, hdims, inp, out, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_dft_r2c" if wide else "plan_guru_dft_r2c"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, out, flags)
+    return Plan(h, prec)
+
+
+def plan_guru_dft_c2r(prec, dims, hdims, inp, out, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_dft_c2r" if wide else "plan_guru_dft_c2r"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, out, flags)
+    return Plan(h, prec)
+
+
+def cdtype(prec):
+    return np.complex128 if prec == "double" else np.complex64
+
+
+def rdtype(prec):
+    return np.float64 if prec == "double" else np.float32

@@ -1,0 +1,702 @@
+/*
+ * fftw3.h API instance.  Compiled twice: once as fftw_ (double) and once
+ * with -DFK_SINGLE as fftwf_ (float).
+ *
+ * Argument mapping follows the reference API layer:
+ *   plan_many_dft      api/plan-many-dft.c:26-51 + api/mktensor-rowmajor.c:23-43
+ *   plan_guru(64)_dft  api/plan-guru-dft.h:24-44 + api/mktensor-iodims.h:23-62
+ *   plan_many_dft_r2c  api/plan-many-dft-r2c.c:24-57 (+ api/rdft2-pad.c:24-39)
+ *   plan_many_dft_c2r  api/plan-many-dft-c2r.c:24-59
+ *   plan_guru_dft_r2c/c2r api/plan-guru-dft-r2c.h:24-43, plan-guru-dft-c2r.h:24-45
+ *   execute*           api/execute*.c
+ *   copy/destroy_plan  api/apiplan.c:180-210
+ * Strides stay in elements of their own array (complex or real), which is
+ * FFTW's user-level convention; the x2 scaling FFTW applies internally is
+ * not needed because the kernels address complex elements directly.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "engine.h"
+#include "../../include/fftw3.h"
+#include "../../include/fftw3_amd.h"
+
+#ifdef FK_SINGLE
+#define X(name) fftwf_##name
+typedef float R;
+typedef fftwf_complex C;
+#define PREC 1
+#else
+#define X(name) fftw_##name
+typedef double R;
+typedef fftw_complex C;
+#define PREC 0
+#endif
+
+typedef X(plan) PLAN;
+
+static PLAN mk(fk_problem *pb)
+{
+    pb->prec = PREC;
+    return (PLAN)fk_plan_create(pb);
+}
+
+static int many_kosher(int rank, const int *n, int howmany)
+{
+    int i;
+    if (howmany < 0 || rank < 0 || rank > FK_MAXRANK) return 0;
+    for (i = 0; i < rank; ++i)
+        if (n[i] <= 0) return 0;
+    return 1;
+}
+
+/* row-major tensor from (n, nembed, stride): api/mktensor-rowmajor.c:23-43 */
+static void rowmajor(int rank, const int *n, const int *ni, const int *no, long long is,
+                     long long os, fk_dim *d)
+{
+    int i;
+    if (rank <= 0) return;
+    d[rank - 1].n = n[rank - 1];
+    d[rank - 1].is = is;
+    d[rank - 1].os = os;
+    for (i = rank - 1; i > 0; --i) {
+        d[i - 1].n = n[i - 1];
+        d[i - 1].is = d[i].is * ni[i];
+        d[i - 1].os = d[i].os * no[i];
+    }
+}
+
+/* ---------------- complex DFT ---------------- */
+
+PLAN X(plan_many_dft)(int rank, const int *n, int howmany, C *in, const int *inembed,
+                      int istride, int idist, C *out, const int *onembed, int ostride, int odist,
+                      int sign, unsigned flags)
+{
+    fk_problem pb;
+    if (!many_kosher(rank, n, howmany)) return 0;
+    memset(&pb, 0, sizeof(pb));
+    pb.kind = FK_C2C;
+    pb.sign = sign == FFTW_FORWARD ? -1 : 1;
+    pb.flags = flags;
+    pb.rank = rank;
+    rowmajor(rank, n, inembed ? inembed : n, onembed ? onembed : n, istride, ostride, pb.dims);
+    pb.vrank = 1;
+    pb.vdims[0].n = howmany;
+    pb.vdims[0].is = idist;
+    pb.vdims[0].os = odist;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_dft)(int rank, const int *n, C *in, C *out, int sign, unsigned flags)
+{
+    return X(plan_many_dft)(rank, n, 1, in, 0, 1, 1, out, 0, 1, 1, sign, flags);
+}
+
+PLAN X(plan_dft_1d)(int n, C *in, C *out, int sign, unsigned flags)
+{
+    return X(plan_dft)(1, &n, in, out, sign, flags);
+}
+
+PLAN X(plan_dft_2d)(int n0, int n1, C *in, C *out, int sign, unsigned flags)
+{
+    int n[2] = {n0, n1};
+    return X(plan_dft)(2, n, in, out, sign, flags);
+}
+
+PLAN X(plan_dft_3d)(int n0, int n1, int n2, C *in, C *out, int sign, unsigned flags)
+{
+    int n[3] = {n0, n1, n2};
+    return X(plan_dft)(3, n, in, out, sign, flags);
+}
+
+/* guru kosherness: api/mktensor-iodims.h:40-62 */
+#define GURU_BODY(DIMT)                                                          \
+    int i;                                                                       \
+    if (rank < 0 || rank > FK_MAXRANK || howmany_rank < 0 ||                     \
+        howmany_rank > FK_MAXRANK)                                               \
+        return 0;                                                                \
+    if (rank > 0 && !dims) return 0;                                             \
+    if (howmany_rank > 0 && !howmany_dims) return 0;                             \
+    for (i = 0; i < rank; ++i)                                                   \
+        if (dims[i].n <= 0) return 0;                                            \
+    for (i = 0; i < howmany_rank; ++i)                                           \
+        if (howmany_dims[i].n < 0) return 0;                                     \
+    memset(&pb, 0, sizeof(pb));                                                  \
+    pb.flags = flags;                                                            \
+    pb.rank = rank;                                                              \
+    pb.vrank = howmany_rank;                                                     \
+    for (i = 0; i < rank; ++i) {                                                 \
+        pb.dims[i].n = dims[i].n;                                                \
+        pb.dims[i].is = dims[i].is;                                              \
+        pb.dims[i].os = dims[i].os;                                              \
+    }                                                                            \
+    for (i = 0; i < howmany_rank; ++i) {                                         \
+        pb.vdims[i].n = howmany_dims[i].n;                                       \
+        pb.vdims[i].is = howmany_dims[i].is;                                     \
+        pb.vdims[i].os = howmany_dims[i].os;                                     \
+    }
+
+PLAN X(plan_guru_dft)(int rank, const X(iodim) *dims, int howmany_rank,
+                      const X(iodim) *howmany_dims, C *in, C *out, int sign, unsigned flags)
+{
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    pb.kind = FK_C2C;
+    pb.sign = sign == FFTW_FORWARD ? -1 : 1;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_guru64_dft)(int rank, const X(iodim64) *dims, int howmany_rank,
+                        const X(iodim64) *howmany_dims, C *in, C *out, int sign, unsigned flags)
+{
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    pb.kind = FK_C2C;
+    pb.sign = sign == FFTW_FORWARD ? -1 : 1;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+/* split-complex arrays are not supported by this engine */
+PLAN X(plan_guru_split_dft)(int rank, const X(iodim) *dims, int howmany_rank,
+                            const X(iodim) *howmany_dims, R *ri, R *ii, R *ro, R *io,
+                            unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)ri; (void)ii; (void)ro; (void)io; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru64_split_dft)(int rank, const X(iodim64) *dims, int howmany_rank,
+                              const X(iodim64) *howmany_dims, R *ri, R *ii, R *ro, R *io,
+                              unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)ri; (void)ii; (void)ro; (void)io; (void)flags;
+    return 0;
+}
+
+static fk_plan *P_(PLAN p) { return (fk_plan *)p; }
+
+void X(execute)(const PLAN p)
+{
+    if (!p) return;
+    fk_execute_or_die(P_(p), P_(p)->in, P_(p)->out);
+}
+
+/* new-array execute: api/execute-dft.c:25-32 (sign is part of the plan) */
+void X(execute_dft)(const PLAN p, C *in, C *out)
+{
+    if (!p) return;
+    fk_execute_or_die(P_(p), in, out);
+}
+
+void X(execute_split_dft)(const PLAN p, R *ri, R *ii, R *ro, R *io)
+{
+    (void)p; (void)ri; (void)ii; (void)ro; (void)io;
+}
+
+/* ---------------- real DFT ---------------- */
+
+/* api/rdft2-pad.c:24-39: default nembed of the padded (in-place) or
+   complex side is n with the last dim n/2+1 complex / 2(n/2+1) real */
+static const int *pad(int rank, const int *n, const int *nembed, int inplace, int cmplx,
+                      int *buf)
+{
+    if (nembed || rank <= 0) return nembed ? nembed : n;
+    if (inplace || cmplx) {
+        memcpy(buf, n, sizeof(int) * rank);
+        buf[rank - 1] = (n[rank - 1] / 2 + 1) * (1 + !cmplx);
+        return buf;
+    }
+    return n;
+}
+
+PLAN X(plan_many_dft_r2c)(int rank, const int *n, int howmany, R *in, const int *inembed,
+                          int istride, int idist, C *out, const int *onembed, int ostride,
+                          int odist, unsigned flags)
+{
+    fk_problem pb;
+    int bi[FK_MAXRANK], bo[FK_MAXRANK], inplace;
+    if (!many_kosher(rank, n, howmany)) return 0;
+    inplace = (void *)in == (void *)out;
+    memset(&pb, 0, sizeof(pb));
+    pb.kind = FK_R2C;
+    pb.sign = -1;
+    pb.flags = flags;
+    pb.rank = rank;
+    rowmajor(rank, n, pad(rank, n, inembed, inplace, 0, bi), pad(rank, n, onembed, inplace, 1, bo),
+             istride, ostride, pb.dims);
+    pb.vrank = 1;
+    pb.vdims[0].n = howmany;
+    pb.vdims[0].is = idist;
+    pb.vdims[0].os = odist;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_dft_r2c)(int rank, const int *n, R *in, C *out, unsigned flags)
+{
+    return X(plan_many_dft_r2c)(rank, n, 1, in, 0, 1, 1, out, 0, 1, 1, flags);
+}
+
+PLAN X(plan_dft_r2c_1d)(int n, R *in, C *out, unsigned flags)
+{
+    return X(plan_dft_r2c)(1, &n, in, out, flags);
+}
+
+PLAN X(plan_dft_r2c_2d)(int n0, int n1, R *in, C *out, unsigned flags)
+{
+    int n[2] = {n0, n1};
+    return X(plan_dft_r2c)(2, n, in, out, flags);
+}
+
+PLAN X(plan_dft_r2c_3d)(int n0, int n1, int n2, R *in, C *out, unsigned flags)
+{
+    int n[3] = {n0, n1, n2};
+    return X(plan_dft_r2c)(3, n, in, out, flags);
+}
+
+PLAN X(plan_many_dft_c2r)(int rank, const int *n, int howmany, C *in, const int *inembed,
+                          int istride, int idist, R *out, const int *onembed, int ostride,
+                          int odist, unsigned flags)
+{
+    fk_problem pb;
+    int bi[FK_MAXRANK], bo[FK_MAXRANK], inplace;
+    if (!many_kosher(rank, n, howmany)) return 0;
+    inplace = (void *)in == (void *)out;
+    if (!inplace) flags |= FFTW_DESTROY_INPUT;
+    memset(&pb, 0, sizeof(pb));
+    pb.kind = FK_C2R;
+    pb.sign = 1;
+    pb.flags = flags;
+    pb.rank = rank;
+    rowmajor(rank, n, pad(rank, n, inembed, inplace, 1, bi), pad(rank, n, onembed, inplace, 0, bo),
+             istride, ostride, pb.dims);
+    pb.vrank = 1;
+    pb.vdims[0].n = howmany;
+    pb.vdims[0].is = idist;
+    pb.vdims[0].os = odist;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_dft_c2r)(int rank, const int *n, C *in, R *out, unsigned flags)
+{
+    return X(plan_many_dft_c2r)(rank, n, 1, in, 0, 1, 1, out, 0, 1, 1, flags);
+}
+
+PLAN X(plan_dft_c2r_1d)(int n, C *in, R *out, unsigned flags)
+{
+    return X(plan_dft_c2r)(1, &n, in, out, flags);
+}
+
+PLAN X(plan_dft_c2r_2d)(int n0, int n1, C *in, R *out, unsigned flags)
+{
+    int n[2] = {n0, n1};
+    return X(plan_dft_c2r)(2, n, in, out, flags);
+}
+
+PLAN X(plan_dft_c2r_3d)(int n0, int n1, int n2, C *in, R *out, unsigned flags)
+{
+    int n[3] = {n0, n1, n2};
+    return X(plan_dft_c2r)(3, n, in, out, flags);
+}
+
+PLAN X(plan_guru_dft_r2c)(int rank, const X(iodim) *dims, int howmany_rank,
+                          const X(iodim) *howmany_dims, R *in, C *out, unsigned flags)
+{
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    pb.kind = FK_R2C;
+    pb.sign = -1;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_guru64_dft_r2c)(int rank, const X(iodim64) *dims, int howmany_rank,
+                            const X(iodim64) *howmany_dims, R *in, C *out, unsigned flags)
+{
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    pb.kind = FK_R2C;
+    pb.sign = -1;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
+}
+
+PLAN X(plan_guru_dft_c2r)(int rank, const X(iodim) *dims, int howmany_rank,
+                          const X(iodim) *howmany_dims, C *in, R *out, unsigned flags)
+{
+    fk_problem pb;
+    if ((void *)out != (void *)in) flags |= FFTW_DESTROY_INPUT;
+    {
+        GURU_BODY(X(iodim))
+        pb.kind = FK_C2R;
+        pb.sign = 1;
+        pb.in = in;
+        pb.out = out;
+    }
+    return mk(&pb);
+}
+
+PLAN X(plan_guru64_dft_c2r)(int rank, const X(iodim64) *dims, int howmany_rank,
+                            const X(iodim64) *howmany_dims, C *in, R *out, unsigned flags)
+{
+    fk_problem pb;
+    if ((void *)out != (void *)in) flags |= FFTW_DESTROY_INPUT;
+    {
+        GURU_BODY(X(iodim64))
+        pb.kind = FK_C2R;
+        pb.sign = 1;
+        pb.in = in;
+        pb.out = out;
+    }
+    return mk(&pb);
+}
+
+PLAN X(plan_guru_split_dft_r2c)(int rank, const X(iodim) *dims, int howmany_rank,
+                                const X(iodim) *howmany_dims, R *in, R *ro, R *io,
+                                unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)in; (void)ro; (void)io; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru_split_dft_c2r)(int rank, const X(iodim) *dims, int howmany_rank,
+                                const X(iodim) *howmany_dims, R *ri, R *ii, R *out,
+                                unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)ri; (void)ii; (void)out; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru64_split_dft_r2c)(int rank, const X(iodim64) *dims, int howmany_rank,
+                                  const X(iodim64) *howmany_dims, R *in, R *ro, R *io,
+                                  unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)in; (void)ro; (void)io; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru64_split_dft_c2r)(int rank, const X(iodim64) *dims, int howmany_rank,
+                                  const X(iodim64) *howmany_dims, R *ri, R *ii, R *out,
+                                  unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
+    (void)ri; (void)ii; (void)out; (void)flags;
+    return 0;
+}
+
+/* api/execute-dft-r2c.c:25-30 / execute-dft-c2r.c:25-30 */
+void X(execute_dft_r2c)(const PLAN p, R *in, C *out)
+{
+    if (!p) return;
+    fk_execute_or_die(P_(p), in, out);
+}
+
+void X(execute_dft_c2r)(const PLAN p, C *in, R *out)
+{
+    if (!p) return;
+    fk_execute_or_die(P_(p), in, out);
+}
+
+void X(execute_split_dft_r2c)(const PLAN p, R *in, R *ro, R *io)
+{
+    (void)p; (void)in; (void)ro; (void)io;
+}
+
+void X(execute_split_dft_c2r)(const PLAN p, R *ri, R *ii, R *out)
+{
+    (void)p; (void)ri; (void)ii; (void)out;
+}
+
+/* ---------------- r2r (not provided: planners return NULL) ---------------- */
+
+PLAN X(plan_many_r2r)(int rank, const int *n, int howmany, R *in, const int *inembed,
+                      int istride, int idist, R *out, const int *onembed, int ostride, int odist,
+                      const X(r2r_kind) *kind, unsigned flags)
+{
+    (void)rank; (void)n; (void)howmany; (void)in; (void)inembed; (void)istride; (void)idist;
+    (void)out; (void)onembed; (void)ostride; (void)odist; (void)kind; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_r2r)(int rank, const int *n, R *in, R *out, const X(r2r_kind) *kind,
+                 unsigned flags)
+{
+    (void)rank; (void)n; (void)in; (void)out; (void)kind; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_r2r_1d)(int n, R *in, R *out, X(r2r_kind) kind, unsigned flags)
+{
+    (void)n; (void)in; (void)out; (void)kind; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_r2r_2d)(int n0, int n1, R *in, R *out, X(r2r_kind) kind0, X(r2r_kind) kind1,
+                    unsigned flags)
+{
+    (void)n0; (void)n1; (void)in; (void)out; (void)kind0; (void)kind1; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_r2r_3d)(int n0, int n1, int n2, R *in, R *out, X(r2r_kind) kind0,
+                    X(r2r_kind) kind1, X(r2r_kind) kind2, unsigned flags)
+{
+    (void)n0; (void)n1; (void)n2; (void)in; (void)out; (void)kind0; (void)kind1; (void)kind2;
+    (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru_r2r)(int rank, const X(iodim) *dims, int howmany_rank,
+                      const X(iodim) *howmany_dims, R *in, R *out, const X(r2r_kind) *kind,
+                      unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims; (void)in; (void)out;
+    (void)kind; (void)flags;
+    return 0;
+}
+
+PLAN X(plan_guru64_r2r)(int rank, const X(iodim64) *dims, int howmany_rank,
+                        const X(iodim64) *howmany_dims, R *in, R *out, const X(r2r_kind) *kind,
+                        unsigned flags)
+{
+    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims; (void)in; (void)out;
+    (void)kind; (void)flags;
+    return 0;
+}
+
+void X(execute_r2r)(const PLAN p, R *in, R *out)
+{
+    (void)p; (void)in; (void)out;
+}
+
+/* ---------------- plan lifetime (api/apiplan.c:180-210) ---------------- */
+
+PLAN X(copy_plan)(PLAN p)
+{
+    if (p) __sync_fetch_and_add(&P_(p)->refcount, 1);
+    return p;
+}
+
+void X(destroy_plan)(PLAN p)
+{
+    if (p && __sync_sub_and_fetch(&P_(p)->refcount, 1) == 0) fk_plan_destroy(P_(p));
+}
+
+void X(forget_wisdom)(void) { fk_forget_wisdom(); }
+void X(cleanup)(void) { fk_cleanup(); }
+void X(set_timelimit)(double t) { fk_set_timelimit(t); }
+
+/* threads API: the device does the parallel work; these keep their
+   contracts (init succeeds, thread count recorded) */
+static int g_nthreads = 1;
+void X(plan_with_nthreads)(int nthreads) { g_nthreads = nthreads < 1 ? 1 : nthreads; }
+int X(planner_nthreads)(void) { return g_nthreads; }
+int X(init_threads)(void) { return 1; }
+void X(cleanup_threads)(void) {}
+void X(threads_set_callback)(void (*parallel_loop)(void *(*work)(char *), char *jobdata,
+                                                   size_t elsize, int njobs, void *data),
+                             void *data)
+{
+    (void)parallel_loop;
+    (void)data;
+}
+void X(make_planner_thread_safe)(void) {}
+
+/* ---------------- wisdom ---------------- */
+
+char *X(export_wisdom_to_string)(void) { return fk_wisdom_export(); }
+
+void X(export_wisdom)(X(write_char_func) write_char, void *data)
+{
+    char *s = fk_wisdom_export(), *q;
+    for (q = s; q && *q; ++q) write_char(*q, data);
+    free(s);
+}
+
+void X(export_wisdom_to_file)(FILE *f)
+{
+    char *s = fk_wisdom_export();
+    if (s) fputs(s, f);
+    free(s);
+}
+
+int X(export_wisdom_to_filename)(const char *filename)
+{
+    FILE *f = fopen(filename, "w");
+    int ok;
+    if (!f) return 0;
+    X(export_wisdom_to_file)(f);
+    ok = !ferror(f);
+    return fclose(f) == 0 && ok;
+}
+
+int X(import_wisdom_from_string)(const char *s) { return s ? fk_wisdom_import(s) : 0; }
+
+int X(import_wisdom)(X(read_char_func) read_char, void *data)
+{
+    size_t cap = 4096, len = 0;
+    char *s = (char *)malloc(cap);
+    int ch, r;
+    while ((ch = read_char(data)) != EOF) {
+        if (len + 2 > cap) s = (char *)realloc(s, cap *= 2);
+        s[len++] = (char)ch;
+    }
+    s[len] = 0;
+    r = fk_wisdom_import(s);
+    free(s);
+    return r;
+}
+
+static int rd_file(void *d) { return fgetc((FILE *)d); }
+
+int X(import_wisdom_from_file)(FILE *f) { return X(import_wisdom)(rd_file, f); }
+
+int X(import_wisdom_from_filename)(const char *filename)
+{
+    FILE *f = fopen(filename, "r");
+    int r;
+    if (!f) return 0;
+    r = X(import_wisdom_from_file)(f);
+    fclose(f);
+    return r;
+}
+
+int X(import_system_wisdom)(void)
+{
+    const char *p = getenv("FFTW_AMD_SYSTEM_WISDOM");
+    return X(import_wisdom_from_filename)(p ? p : "/etc/fftw/wisdom-gfx950");
+}
+
+/* ---------------- printing ---------------- */
+
+void X(fprint_plan)(const PLAN p, FILE *f) { fk_plan_print(P_(p), f); }
+void X(print_plan)(const PLAN p) { fk_plan_print(P_(p), stdout); }
+char *X(sprint_plan)(const PLAN p) { return fk_plan_sprint(P_(p)); }
+
+/* ---------------- memory: device allocations ---------------- */
+
+void *X(malloc)(size_t n) { return fk_malloc(n); }
+R *X(alloc_real)(size_t n) { return (R *)fk_malloc(sizeof(R) * n); }
+C *X(alloc_complex)(size_t n) { return (C *)fk_malloc(sizeof(C) * n); }
+void X(free)(void *p) { fk_free(p); }
+
+/* ---------------- costs ---------------- */
+
+void X(flops)(const PLAN p, double *add, double *mul, double *fmas)
+{
+    *add = p ? P_(p)->adds : 0;
+    *mul = p ? P_(p)->muls : 0;
+    *fmas = p ? P_(p)->fmas : 0;
+}
+
+double X(estimate_cost)(const PLAN p)
+{
+    /* HBM-bound model: bytes moved at 8 TB/s, in seconds */
+    return p ? fk_plan_bytes(P_(p)) / 8.0e12 : 0;
+}
+
+double X(cost)(const PLAN p) { return p ? P_(p)->pcost : 0; }
+
+int X(alignment_of)(R *p) { return (int)((uintptr_t)p % 16); }
+
+#ifdef FK_SINGLE
+const char fftwf_version[] = "fftw-3.3-compatible fftw3-amd-gfx950-0.1";
+const char fftwf_cc[] = "hipcc --offload-arch=gfx950";
+const char fftwf_codelet_optim[] = "stockham-lds";
+#else
+const char fftw_version[] = "fftw-3.3-compatible fftw3-amd-gfx950-0.1";
+const char fftw_cc[] = "hipcc --offload-arch=gfx950";
+const char fftw_codelet_optim[] = "stockham-lds";
+#endif
+
+/* ---------------- extensions (defined once, in the double instance) ---- */
+#ifndef FK_SINGLE
+int fftw_amd_init(void) { return fk_init(); }
+const char *fftw_amd_last_error(void) { return fk_last_error(); }
+void fftw_amd_set_stream(void *s) { fk_set_stream((hipStream_t)s); }
+void *fftw_amd_get_stream(void) { return (void *)fk_get_stream(); }
+void fftw_amd_plan_set_stream(void *p, void *s) { fk_plan_set_stream((fk_plan *)p, (hipStream_t)s); }
+void fftw_amd_set_async(int on) { fk_set_async(on); }
+
+int fftw_amd_memcpy_htod(void *d, const void *s, size_t b)
+{
+    return (int)hipMemcpy(d, s, b, hipMemcpyHostToDevice);
+}
+int fftw_amd_memcpy_dtoh(void *d, const void *s, size_t b)
+{
+    return (int)hipMemcpy(d, s, b, hipMemcpyDeviceToHost);
+}
+int fftw_amd_memcpy_dtod(void *d, const void *s, size_t b)
+{
+    return (int)hipMemcpy(d, s, b, hipMemcpyDeviceToDevice);
+}
+int fftw_amd_memset(void *d, int v, size_t b) { return (int)hipMemset(d, v, b); }
+int fftw_amd_synchronize(void) { return (int)hipDeviceSynchronize(); }
+
+size_t fftw_amd_mem_free(void)
+{
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+    return f;
+}
+
+int fftw_amd_fill_uniform(void *dev, int prec, size_t count, uint64_t seed, uint64_t stream)
+{
+    int rc = fkm_fill(dev, prec, (long long)count, seed, stream, 0, fk_get_stream());
+    return rc ? rc : (int)hipStreamSynchronize(fk_get_stream());
+}
+
+int fftw_amd_fill_gauss(void *dev, int prec, size_t count, uint64_t seed, uint64_t stream)
+{
+    int rc = fkm_fill(dev, prec, (long long)count, seed, stream, 1, fk_get_stream());
+    return rc ? rc : (int)hipStreamSynchronize(fk_get_stream());
+}
+
+int fftw_amd_plan_npasses(const void *p) { return fk_plan_npasses((const fk_plan *)p); }
+int fftw_amd_plan_pass_info(const void *p, int i, char *buf, int len)
+{
+    return fk_plan_pass_info((const fk_plan *)p, i, buf, len);
+}
+double fftw_amd_plan_bytes(const void *p) { return fk_plan_bytes((const fk_plan *)p); }
+double fftw_amd_plan_pass_bytes(const void *p, int i)
+{
+    const fk_plan *q = (const fk_plan *)p;
+    return q && i >= 0 && i < q->npass ? q->pass[i].bytes : -1;
+}
+double fftw_amd_plan_time(void *p, void *in, void *out, int iters, double *per_pass_ms)
+{
+    return p ? fk_plan_time((fk_plan *)p, in, out, iters, per_pass_ms) : -1;
+}
+
+void *fftw_amd_event_create(void)
+{
+    hipEvent_t e = 0;
+    if (hipEventCreate(&e) != hipSuccess) return 0;
+    return (void *)e;
+}
+void fftw_amd_event_record(void *ev, void *s) { (void)hipEventRecord((hipEvent_t)ev, (hipStream_t)s); }
+double fftw_amd_event_elapsed(void *a, void *b)
+{
+    float ms = -1;
+    if (hipEventSynchronize((hipEvent_t)b) != hipSuccess) return -1;
+    if (hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b) != hipSuccess) return -1;
+    return ms;
+}
+void fftw_amd_event_destroy(void *ev) { (void)hipEventDestroy((hipEvent_t)ev); }
+#endif

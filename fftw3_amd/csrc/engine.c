@@ -1,0 +1,1347 @@
+/*
+ * Host engine: problem -> pass list, device tables, execution.
+ *
+ * Decomposition rules (what FFTW's solvers do, re-planned for a GPU):
+ *   - multi-D c2c: one batched 1-D transform per dimension, later ones in
+ *     place on the output (dft/rank-geq2.c:42-52);
+ *   - batch dimensions ride inside each kernel launch as up to three
+ *     strided batch dims (dft/vrank-geq1.c:54-65);
+ *   - 1-D length with a catalog kernel: one fused Stockham pass;
+ *   - other smooth lengths n = n1*n2: column pass of n1 with the twiddle
+ *     W_n^{k1*n2} fused into its store, then n2-point transforms written
+ *     transposed (Cooley-Tukey, dft/ct.c:34-45, as a two-pass four-step);
+ *   - lengths with large prime factors: Bluestein with a power-of-two
+ *     convolution (dft/bluestein.c:82-128), fused into one kernel when the
+ *     convolution fits one workgroup;
+ *   - r2c: half-length complex transform + post-pass (rdft/ct-hc2c.c:
+ *     197-210), then the leading dims as c2c on n/2+1 columns
+ *     (rdft/rank-geq2-rdft2.c:40-53); c2r mirrors it (:55-69).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "engine.h"
+
+/* ------------------------------------------------------------------ */
+/* globals                                                             */
+
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+static const fk_kernel *g_reg = 0;
+static int g_nreg = 0;
+static int g_inited = 0;
+static hipStream_t g_stream = 0;
+static int g_async = 0;
+static double g_timelimit = -1.0;
+static __thread char g_err[512];
+
+static void set_err(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *fk_last_error(void) { return g_err; }
+void fk_set_stream(hipStream_t s) { g_stream = s; }
+hipStream_t fk_get_stream(void) { return g_stream; }
+void fk_set_async(int on) { g_async = on; }
+int fk_get_async(void) { return g_async; }
+void fk_set_timelimit(double t) { g_timelimit = t; }
+
+int fk_init(void)
+{
+    int i, ndev = 0;
+    if (g_inited) return 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_err("no HIP device available");
+        return -1;
+    }
+    g_reg = fk_registry(&g_nreg);
+    for (i = 0; i < g_nreg; ++i) {
+        if (g_reg[i].lds > 65536) {
+            if (hipFuncSetAttribute(g_reg[i].fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)g_reg[i].lds) != hipSuccess) {
+                set_err("hipFuncSetAttribute failed for L=%d", g_reg[i].L);
+                (void)hipGetLastError();
+            }
+        }
+    }
+    g_inited = 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* device memory                                                       */
+
+void *fk_malloc(size_t n)
+{
+    void *p = 0;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return p;
+}
+
+void fk_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+/* ------------------------------------------------------------------ */
+/* cached device tables (freed by fk_cleanup)                          */
+
+enum { T_STAGE, T_TW2, T_LINE, T_CHIRP, T_BLUEB };
+
+typedef struct tbl {
+    int type, prec, nrad;
+    long long a, b;
+    int rad[12];
+    void *dev;
+    size_t bytes;
+    struct tbl *next;
+} tbl;
+
+static tbl *g_tbl = 0;
+
+static tbl *tbl_find(int type, int prec, long long a, long long b, int nrad, const int *rad)
+{
+    tbl *t;
+    for (t = g_tbl; t; t = t->next) {
+        if (t->type != type || t->prec != prec || t->a != a || t->b != b || t->nrad != nrad)
+            continue;
+        if (nrad && memcmp(t->rad, rad, sizeof(int) * nrad)) continue;
+        return t;
+    }
+    return 0;
+}
+
+static tbl *tbl_new(int type, int prec, long long a, long long b, int nrad, const int *rad,
+                    size_t bytes)
+{
+    tbl *t = (tbl *)calloc(1, sizeof(tbl));
+    t->type = type;
+    t->prec = prec;
+    t->a = a;
+    t->b = b;
+    t->nrad = nrad;
+    if (nrad) memcpy(t->rad, rad, sizeof(int) * nrad);
+    t->bytes = bytes;
+    t->dev = fk_malloc(bytes);
+    if (!t->dev) {
+        free(t);
+        return 0;
+    }
+    t->next = g_tbl;
+    g_tbl = t;
+    return t;
+}
+
+static size_t csz(int prec) { return prec == 0 ? 16 : 8; }
+static size_t rsz(int prec) { return prec == 0 ? 8 : 4; }
+
+static const void *stage_table(const fk_kernel *k, hipStream_t st)
+{
+    tbl *t;
+    if (k->twsz == 0) return 0;
+    t = tbl_find(T_STAGE, k->prec, k->L, 0, k->nrad, k->rad);
+    if (t) return t->dev;
+    t = tbl_new(T_STAGE, k->prec, k->L, 0, k->nrad, k->rad, csz(k->prec) * k->twsz);
+    if (!t) return 0;
+    if (fkm_gen_stage_tw(t->dev, k->prec, k->nrad, k->rad, k->twsz, st)) return 0;
+    return t->dev;
+}
+
+static long long isqrt_ceil(long long n)
+{
+    long long b = (long long)sqrt((double)n);
+    while (b * b < n) ++b;
+    while (b > 1 && (b - 1) * (b - 1) >= n) --b;
+    return b < 1 ? 1 : b;
+}
+
+/* FP64 two-level table of W_N^e: lo (b entries) then hi (N/b+1 entries) */
+static const double *tw2_table(long long N, long long *b_out, hipStream_t st)
+{
+    long long b = isqrt_ceil(N);
+    tbl *t = tbl_find(T_TW2, 0, N, b, 0, 0);
+    *b_out = b;
+    if (t) return (const double *)t->dev;
+    t = tbl_new(T_TW2, 0, N, b, 0, 0, 16 * (size_t)(b + N / b + 1));
+    if (!t) return 0;
+    if (fkm_gen_tw2((double *)t->dev, (double *)t->dev + 2 * b, N, b, st)) return 0;
+    return (const double *)t->dev;
+}
+
+static const void *line_table(int prec, long long n, long long count, hipStream_t st)
+{
+    tbl *t = tbl_find(T_LINE, prec, n, count, 0, 0);
+    if (t) return t->dev;
+    t = tbl_new(T_LINE, prec, n, count, 0, 0, csz(prec) * (size_t)count);
+    if (!t) return 0;
+    if (fkm_gen_line_tw(t->dev, prec, n, count, st)) return 0;
+    return t->dev;
+}
+
+static const void *chirp_table(int prec, long long n, hipStream_t st)
+{
+    tbl *t = tbl_find(T_CHIRP, prec, n, 0, 0, 0);
+    if (t) return t->dev;
+    t = tbl_new(T_CHIRP, prec, n, 0, 0, 0, csz(prec) * (size_t)n);
+    if (!t) return 0;
+    if (fkm_gen_chirp(t->dev, prec, n, st)) return 0;
+    return t->dev;
+}
+
+static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st);
+
+/* B = DFT_M(conj chirp)/M, computed with an FP64 plan of the engine itself
+   and rounded to the working precision */
+static const void *blue_table(int prec, long long n, long long M, hipStream_t st)
+{
+    tbl *t = tbl_find(T_BLUEB, prec, n, M, 0, 0);
+    fk_problem pb;
+    fk_plan *sub;
+    void *b64, *B64;
+    int rc;
+    if (t) return t->dev;
+    b64 = fk_malloc(16 * (size_t)M);
+    B64 = fk_malloc(16 * (size_t)M);
+    if (!b64 || !B64) goto fail;
+    memset(&pb, 0, sizeof(pb));
+    pb.kind = FK_C2C;
+    pb.prec = 0;
+    pb.sign = -1;
+    pb.flags = FK_ESTIMATE;
+    pb.rank = 1;
+    pb.dims[0].n = M;
+    pb.dims[0].is = pb.dims[0].os = 1;
+    pb.in = b64;
+    pb.out = B64;
+    sub = plan_internal(&pb, st);
+    if (!sub) goto fail;
+    sub->stream = st;
+    rc = fkm_gen_blue_b(b64, n, M, st);
+    if (!rc) rc = fk_execute(sub, b64, B64);
+    if (rc) {
+        fk_plan_destroy(sub);
+        goto fail;
+    }
+    t = tbl_new(T_BLUEB, prec, n, M, 0, 0, csz(prec) * (size_t)M);
+    if (t) rc = fkm_scale_round(t->dev, prec, B64, M, 1.0 / (double)M, st);
+    (void)hipStreamSynchronize(st);
+    fk_plan_destroy(sub);
+    fk_free(b64);
+    fk_free(B64);
+    return t && !rc ? t->dev : 0;
+fail:
+    fk_free(b64);
+    fk_free(B64);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* kernel catalog queries                                              */
+
+static const fk_kernel *find_kernel(int prec, long long L, int mode, int io)
+{
+    int i;
+    if (L > (1 << 20)) return 0;
+    for (i = 0; i < g_nreg; ++i) {
+        const fk_kernel *k = g_reg + i;
+        if (k->prec == prec && k->L == L && k->mode == mode && k->io == io) return k;
+    }
+    return 0;
+}
+
+static int has_single(int prec, long long n) { return find_kernel(prec, n, FK_ROW, FK_IO_PLAIN) != 0; }
+
+/* n decomposable into catalog passes without Bluestein */
+static int smooth_plannable(int prec, long long n, int depth)
+{
+    long long n1;
+    if (n == 1 || has_single(prec, n)) return 1;
+    if (depth >= 3) return 0;
+    for (n1 = 2; n1 <= 16384 && n1 < n; ++n1) {
+        if (n % n1) continue;
+        if (!find_kernel(prec, n1, FK_COL, FK_IO_PLAIN)) continue;
+        if (smooth_plannable(prec, n / n1, depth + 1)) return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* plan building                                                       */
+
+typedef struct {
+    fk_plan *P;
+    hipStream_t st;
+    int err;
+} ctx;
+
+static fk_pass *new_pass(ctx *c)
+{
+    fk_plan *P = c->P;
+    if (P->npass == P->cpass) {
+        P->cpass = P->cpass ? 2 * P->cpass : 8;
+        P->pass = (fk_pass *)realloc(P->pass, sizeof(fk_pass) * P->cpass);
+    }
+    memset(P->pass + P->npass, 0, sizeof(fk_pass));
+    return P->pass + P->npass++;
+}
+
+static void need_scratch(ctx *c, int slot, size_t bytes)
+{
+    fk_plan *P = c->P;
+    if (slot >= FK_MAX_SCRATCH) {
+        c->err = 1;
+        set_err("too many scratch buffers");
+        return;
+    }
+    if (P->scratch_bytes[slot] < bytes) P->scratch_bytes[slot] = bytes;
+    if (P->nscratch < slot + 1) P->nscratch = slot + 1;
+}
+
+static long long llabs_(long long x) { return x < 0 ? -x : x; }
+
+/* Drop unit dims; stable. */
+static int squeeze(int nb, fk_dim *b)
+{
+    int i, m = 0;
+    for (i = 0; i < nb; ++i)
+        if (b[i].n != 1) b[m++] = b[i];
+    return m;
+}
+
+static void sort_by_is(int nb, fk_dim *b)
+{
+    int i, j;
+    for (i = 1; i < nb; ++i)
+        for (j = i; j > 0 && llabs_(b[j].is) < llabs_(b[j - 1].is); --j) {
+            fk_dim t = b[j];
+            b[j] = b[j - 1];
+            b[j - 1] = t;
+        }
+}
+
+/* Merge batch dims that describe one contiguous run on both sides. */
+static int merge_dims(int nb, fk_dim *b)
+{
+    int i, changed = 1;
+    while (changed) {
+        changed = 0;
+        for (i = 0; i + 1 < nb && !changed; ++i) {
+            int j;
+            for (j = 0; j < nb; ++j) {
+                if (j == i) continue;
+                /* dim j directly outside dim i? */
+                if (b[j].is == b[i].is * b[i].n && b[j].os == b[i].os * b[i].n) {
+                    b[i].n *= b[j].n;
+                    memmove(b + j, b + j + 1, sizeof(fk_dim) * (nb - j - 1));
+                    --nb;
+                    changed = 1;
+                    break;
+                }
+            }
+        }
+    }
+    return nb;
+}
+
+/* Emit one pass per combination of batch dims beyond the three a kernel
+   addresses itself. */
+typedef void (*emit_fn)(ctx *c, fk_pass *proto, int nb, const fk_dim *b, long long ioff,
+                        long long ooff);
+
+static void fill_batch3(long long *nbv, long long *ids, long long *ods, long long *nl, int nb,
+                        const fk_dim *b)
+{
+    int i;
+    long long t = 1;
+    for (i = 0; i < 3; ++i) {
+        if (i < nb) {
+            nbv[i] = b[i].n;
+            ids[i] = b[i].is;
+            ods[i] = b[i].os;
+        } else {
+            nbv[i] = 1;
+            ids[i] = ods[i] = 0;
+        }
+        t *= nbv[i];
+    }
+    *nl = t;
+}
+
+static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b, size_t isz,
+                       size_t osz)
+{
+    long long idx[FK_MAXRANK * 2] = {0};
+    int nloop = nb > 3 ? nb - 3 : 0, i;
+    for (;;) {
+        long long ioff = proto->ioff, ooff = proto->ooff;
+        fk_pass *p;
+        for (i = 0; i < nloop; ++i) {
+            ioff += idx[i] * b[3 + i].is * (long long)isz;
+            ooff += idx[i] * b[3 + i].os * (long long)osz;
+        }
+        p = new_pass(c);
+        *p = *proto;
+        p->ioff = ioff;
+        p->ooff = ooff;
+        if (p->type == FK_PASS_STOCKHAM) {
+            fill_batch3(p->a.nb, p->a.ids, p->a.ods, &p->a.nlines, nb < 3 ? nb : 3, b);
+            p->grid = (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
+            if (p->a.nlines == 0) --c->P->npass;
+        } else {
+            fill_batch3(p->e.nb, p->e.ids, p->e.ods, &p->e.nlines, nb < 3 ? nb : 3, b);
+            if (p->e.nlines == 0) --c->P->npass;
+        }
+        for (i = 0; i < nloop; ++i) {
+            if (++idx[i] < b[3 + i].n) break;
+            idx[i] = 0;
+        }
+        if (i == nloop) break;
+    }
+}
+
+static double pass_bytes_c(int prec, double n_elems) { return 2.0 * n_elems * csz(prec); }
+
+static long long batch_count(int nb, const fk_dim *b)
+{
+    long long t = 1;
+    int i;
+    for (i = 0; i < nb; ++i) t *= b[i].n;
+    return t;
+}
+
+static void emit_stockham(ctx *c, const fk_kernel *k, long long is, long long os, int nb,
+                          const fk_dim *b, int ibuf, long long ioff, int obuf, long long ooff,
+                          unsigned flags, size_t isz, size_t osz)
+{
+    fk_pass proto;
+    memset(&proto, 0, sizeof(proto));
+    proto.type = FK_PASS_STOCKHAM;
+    proto.k = k;
+    proto.ibuf = ibuf;
+    proto.obuf = obuf;
+    proto.ioff = ioff;
+    proto.ooff = ooff;
+    proto.block = (unsigned)(k->tpl * k->nl);
+    proto.lds = k->lds;
+    proto.a.is = is;
+    proto.a.os = os;
+    proto.a.flags = flags;
+    proto.a.tw = stage_table(k, c->st);
+    if (k->twsz && !proto.a.tw) {
+        c->err = 1;
+        set_err("twiddle table allocation failed");
+        return;
+    }
+    proto.a.xcd_remap = (k->mode == FK_COL || k->io == FK_IO_TOUT) ? 1u : 0u;
+    proto.bytes = pass_bytes_c(k->prec, (double)k->L * (double)batch_count(nb, b));
+    emit_loops(c, &proto, nb, b, isz, osz);
+}
+
+static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb,
+                       const fk_dim *bin, int ib, long long io, int ob, long long oo, int swi,
+                       int swo, int slot);
+
+/* tw: FK_TW_OUT setup helper */
+static int set_tw_out(ctx *c, fk_pass *proto, long long N, long long twm)
+{
+    long long b;
+    const double *t = tw2_table(N, &b, c->st);
+    if (!t) return -1;
+    proto->a.tlo = t;
+    proto->a.thi = t + 2 * b;
+    proto->a.tN = N;
+    proto->a.tb = b;
+    proto->a.twm = twm;
+    return 0;
+}
+
+static int plan_bluestein(ctx *c, long long n, long long is, long long os, int nb,
+                          const fk_dim *b, int ib, long long io, int ob, long long oo, int swi,
+                          int swo, int slot)
+{
+    int prec = c->P->prec;
+    long long M = 1;
+    const fk_kernel *kb;
+    const void *chirp, *B;
+    fk_dim bb[FK_MAXRANK * 2];
+    size_t cs = csz(prec);
+    while (M < 2 * n - 1) M <<= 1;
+    chirp = chirp_table(prec, n, c->st);
+    B = blue_table(prec, n, M, c->st);
+    if (!chirp || !B) {
+        c->err = 1;
+        set_err("Bluestein tables failed (n=%lld)", n);
+        return -1;
+    }
+    memcpy(bb, b, sizeof(fk_dim) * nb);
+    kb = find_kernel(prec, M, FK_ROW, FK_IO_BLUE);
+    if (kb) {
+        size_t before = (size_t)c->P->npass;
+        unsigned fl = (swi ? FK_SWAP_IN : 0) | (swo ? FK_SWAP_OUT : 0);
+        emit_stockham(c, kb, is, os, nb, bb, ib, io, ob, oo, fl, cs, cs);
+        for (; before < (size_t)c->P->npass; ++before) {
+            fk_pass *p = c->P->pass + before;
+            p->a.aux = chirp;
+            p->a.aux2 = B;
+            p->a.nvalid = n;
+            p->bytes = pass_bytes_c(prec, (double)n * (double)batch_count(nb, b));
+        }
+        return c->err ? -1 : 0;
+    } else {
+        /* multi-pass: prep -> DFT_M -> pointwise -> DFT_M -> post */
+        long long H = batch_count(nb, b), s;
+        fk_dim sb[FK_MAXRANK * 2];
+        fk_pass proto;
+        int i, s1 = slot, s2 = slot + 1;
+        need_scratch(c, s1, cs * (size_t)M * (size_t)H);
+        need_scratch(c, s2, cs * (size_t)M * (size_t)H);
+        /* scratch batch layout: contiguous [h][M] in the order of b */
+        s = M;
+        for (i = 0; i < nb; ++i) {
+            sb[i].n = b[i].n;
+            sb[i].is = b[i].is;
+            sb[i].os = s;
+            s *= b[i].n;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_BS_PREP;
+        proto.e.n = M;
+        proto.e.is = is;
+        proto.e.os = 1;
+        proto.e.aux = chirp;
+        proto.e.p0 = n;
+        proto.e.p1 = M;
+        proto.e.flags = swi ? FK_SWAP_IN : 0;
+        proto.ibuf = ib;
+        proto.ioff = io;
+        proto.obuf = FK_BUF_S0 + s1;
+        proto.bytes = (double)cs * (double)(n + M) * (double)H;
+        emit_loops(c, &proto, nb, sb, cs, cs);
+        for (i = 0; i < nb; ++i) sb[i].is = sb[i].os;
+        if (plan_c2c_1d(c, M, 1, 1, nb, sb, FK_BUF_S0 + s1, 0, FK_BUF_S0 + s2, 0, 0, 0, slot + 2))
+            return -1;
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_BS_MUL;
+        proto.e.n = M;
+        proto.e.is = proto.e.os = 1;
+        proto.e.aux2 = B;
+        proto.ibuf = proto.obuf = FK_BUF_S0 + s2;
+        proto.bytes = 2.0 * (double)cs * (double)M * (double)H;
+        emit_loops(c, &proto, nb, sb, cs, cs);
+        if (plan_c2c_1d(c, M, 1, 1, nb, sb, FK_BUF_S0 + s2, 0, FK_BUF_S0 + s1, 0, 0, 0, slot + 2))
+            return -1;
+        for (i = 0; i < nb; ++i) {
+            sb[i].is = sb[i].os;
+            sb[i].os = b[i].os;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_BS_POST;
+        proto.e.n = n;
+        proto.e.is = 1;
+        proto.e.os = os;
+        proto.e.aux = chirp;
+        proto.e.p0 = n;
+        proto.e.flags = swo ? FK_SWAP_OUT : 0;
+        proto.ibuf = FK_BUF_S0 + s1;
+        proto.obuf = ob;
+        proto.ooff = oo;
+        proto.bytes = 2.0 * (double)cs * (double)n * (double)H;
+        emit_loops(c, &proto, nb, sb, cs, cs);
+        return c->err ? -1 : 0;
+    }
+}
+
+/* pick n1 (column pass length) for a two-pass split of n */
+static long long pick_split(int prec, long long n)
+{
+    long long n1, best = 0;
+    double bestscore = 1e300;
+    for (n1 = 2; n1 <= 16384 && n1 < n; ++n1) {
+        long long n2;
+        double score;
+        if (n % n1) continue;
+        n2 = n / n1;
+        if (!find_kernel(prec, n1, FK_COL, FK_IO_PLAIN)) continue;
+        if (!smooth_plannable(prec, n2, 1)) continue;
+        score = fabs(log((double)n1) - log((double)n2));
+        if (!has_single(prec, n2)) score += 100.0;
+        if (n1 < 16) score += 10.0;
+        if (score < bestscore || (score == bestscore && n2 > n / best)) {
+            bestscore = score;
+            best = n1;
+        }
+    }
+    return best;
+}
+
+/*
+ * Batched 1-D complex DFT of length n: element j of batch line h at
+ * h-offset + j*is (input) / j*os (output).  Strides in complex elements.
+ */
+static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
+                       const fk_dim *bin, int ib, long long io, int ob, long long oo, int swi,
+                       int swo, int slot)
+{
+    int prec = c->P->prec, nb, i;
+    size_t cs = csz(prec);
+    fk_dim b[FK_MAXRANK * 2];
+    const fk_kernel *kr, *kc, *kt;
+    unsigned fl = (swi ? FK_SWAP_IN : 0) | (swo ? FK_SWAP_OUT : 0);
+    memcpy(b, bin, sizeof(fk_dim) * nb_);
+    nb = squeeze(nb_, b);
+    nb = merge_dims(nb, b);
+    sort_by_is(nb, b);
+    if (n == 1) {
+        /* length-1 DFT is a copy (with the re/im swaps cancelling) */
+        fk_pass proto;
+        if (ib == ob && io == oo && is == os) {
+            int same = 1;
+            for (i = 0; i < nb; ++i) same &= b[i].is == b[i].os;
+            if (same) return 0;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_COPY;
+        proto.e.n = 1;
+        proto.e.is = is;
+        proto.e.os = os;
+        proto.e.flags = fl;
+        proto.ibuf = ib;
+        proto.ioff = io;
+        proto.obuf = ob;
+        proto.ooff = oo;
+        proto.bytes = 2.0 * cs * (double)batch_count(nb, b);
+        emit_loops(c, &proto, nb, b, cs, cs);
+        return 0;
+    }
+    kr = find_kernel(prec, n, FK_ROW, FK_IO_PLAIN);
+    if (kr) {
+        kc = find_kernel(prec, n, FK_COL, FK_IO_PLAIN);
+        kt = find_kernel(prec, n, FK_ROW, FK_IO_TOUT);
+        if (llabs_(is) == 1) {
+            /* transposed store when the output line stride is large and some
+               batch dim is unit-stride on output */
+            int d = -1;
+            if (kt && llabs_(os) > 1)
+                for (i = 0; i < nb; ++i)
+                    if (llabs_(b[i].os) == 1) { d = i; break; }
+            if (d >= 0) {
+                fk_dim t = b[d];
+                memmove(b + 1, b, sizeof(fk_dim) * d);
+                b[0] = t;
+                emit_stockham(c, kt, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
+            } else {
+                emit_stockham(c, kr, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
+            }
+        } else {
+            int d = -1;
+            long long best = 1LL << 62;
+            for (i = 0; i < nb; ++i) {
+                long long s = llabs_(b[i].is) + llabs_(b[i].os);
+                if (s < best) { best = s; d = i; }
+            }
+            if (kc && d >= 0 && best <= 4 && b[d].n >= 2) {
+                fk_dim t = b[d];
+                memmove(b + 1, b, sizeof(fk_dim) * d);
+                b[0] = t;
+                emit_stockham(c, kc, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
+            } else {
+                emit_stockham(c, kr, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
+            }
+        }
+        return c->err ? -1 : 0;
+    }
+    if (smooth_plannable(prec, n, 0)) {
+        long long n1 = pick_split(prec, n), n2, H, s;
+        fk_dim ab[FK_MAXRANK * 2 + 1], bb[FK_MAXRANK * 2 + 1];
+        fk_pass proto;
+        const fk_kernel *kA;
+        size_t before;
+        if (!n1) goto blue;
+        n2 = n / n1;
+        kA = find_kernel(prec, n1, FK_COL, FK_IO_PLAIN);
+        H = batch_count(nb, b);
+        need_scratch(c, slot, cs * (size_t)n * (size_t)H);
+        /* pass A: lines n2_idx (tile dim), elements n1_idx at stride is*n2 */
+        ab[0].n = n2;
+        ab[0].is = is;
+        ab[0].os = 1;
+        s = n;
+        for (i = 0; i < nb; ++i) {
+            ab[i + 1].n = b[i].n;
+            ab[i + 1].is = b[i].is;
+            ab[i + 1].os = s;
+            s *= b[i].n;
+        }
+        memset(&proto, 0, sizeof(proto));
+        before = (size_t)c->P->npass;
+        emit_stockham(c, kA, is * n2, n2, nb + 1, ab, ib, io, FK_BUF_S0 + slot, 0,
+                      (swi ? FK_SWAP_IN : 0) | FK_TW_OUT, cs, cs);
+        if (c->err) return -1;
+        {
+            long long tb;
+            const double *t = tw2_table(n, &tb, c->st);
+            if (!t) { c->err = 1; return -1; }
+            for (; before < (size_t)c->P->npass; ++before) {
+                fk_pass *p = c->P->pass + before;
+                p->a.tlo = t;
+                p->a.thi = t + 2 * tb;
+                p->a.tN = n;
+                p->a.tb = tb;
+                p->a.twm = 1;
+            }
+        }
+        /* pass B: n2-point transforms of rows k1, written to k1 + n1*k2 */
+        bb[0].n = n1;
+        bb[0].is = n2;
+        bb[0].os = os;
+        for (i = 0; i < nb; ++i) {
+            bb[i + 1].n = b[i].n;
+            bb[i + 1].is = ab[i + 1].os;
+            bb[i + 1].os = b[i].os;
+        }
+        return plan_c2c_1d(c, n2, 1, n1 * os, nb + 1, bb, FK_BUF_S0 + slot, 0, ob, oo, 0, swo,
+                           slot + 1);
+    }
+blue:
+    return plan_bluestein(c, n, is, os, nb, b, ib, io, ob, oo, swi, swo, slot);
+}
+
+/* ------------------------------------------------------------------ */
+/* problem-level planning                                              */
+
+static void desc_add(fk_plan *P, const char *fmt, ...)
+{
+    size_t l = strlen(P->desc);
+    va_list ap;
+    if (l + 8 >= sizeof(P->desc)) return;
+    va_start(ap, fmt);
+    vsnprintf(P->desc + l, sizeof(P->desc) - l, fmt, ap);
+    va_end(ap);
+}
+
+/* c2c over all transform dims, from (ib) to (ob), then in place on ob */
+static int plan_c2c_dims(ctx *c, int rank, const fk_dim *dims, int vrank, const fk_dim *vdims,
+                         int ib, int ob, int sw)
+{
+    int d, cur = ib, i;
+    for (d = rank - 1; d >= 0; --d) {
+        fk_dim b[FK_MAXRANK * 2];
+        int nb = 0;
+        long long is = cur == ib ? dims[d].is : dims[d].os;
+        for (i = 0; i < rank; ++i) {
+            if (i == d) continue;
+            b[nb].n = dims[i].n;
+            b[nb].is = cur == ib ? dims[i].is : dims[i].os;
+            b[nb].os = dims[i].os;
+            ++nb;
+        }
+        for (i = 0; i < vrank; ++i) {
+            b[nb].n = vdims[i].n;
+            b[nb].is = cur == ib ? vdims[i].is : vdims[i].os;
+            b[nb].os = vdims[i].os;
+            ++nb;
+        }
+        if (plan_c2c_1d(c, dims[d].n, is, dims[d].os, nb, b, cur, 0, ob, 0, sw, sw, 0))
+            return -1;
+        cur = ob;
+    }
+    return 0;
+}
+
+static int plan_copy(ctx *c, int vrank, const fk_dim *vdims, int op, size_t esz)
+{
+    fk_pass proto;
+    fk_dim b[FK_MAXRANK];
+    int nb;
+    memcpy(b, vdims, sizeof(fk_dim) * vrank);
+    nb = squeeze(vrank, b);
+    nb = merge_dims(nb, b);
+    sort_by_is(nb, b);
+    memset(&proto, 0, sizeof(proto));
+    proto.type = FK_PASS_EW;
+    proto.e.op = op;
+    proto.e.n = 1;
+    proto.e.is = proto.e.os = 0;
+    proto.ibuf = FK_BUF_IN;
+    proto.obuf = FK_BUF_OUT;
+    proto.bytes = 2.0 * (double)esz * (double)batch_count(nb, b);
+    emit_loops(c, &proto, nb, b, esz, esz);
+    return 0;
+}
+
+static int plan_r2c(ctx *c, const fk_problem *pb)
+{
+    fk_plan *P = c->P;
+    int prec = P->prec, r = pb->rank, i;
+    size_t cs = csz(prec), rs = rsz(prec);
+    long long nl = pb->dims[r - 1].n, nh = nl / 2 + 1, L = nl / 2;
+    fk_dim b[FK_MAXRANK * 2], lead[FK_MAXRANK];
+    int nb = 0;
+    const fk_kernel *k = 0;
+    /* batch for the last-dim pass: leading dims + vdims */
+    for (i = 0; i < r - 1; ++i) b[nb++] = pb->dims[i];
+    for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
+    nb = squeeze(nb, b);
+    sort_by_is(nb, b);
+    if (nl % 2 == 0) k = find_kernel(prec, L, FK_ROW, FK_IO_R2C);
+    if (k) {
+        size_t before = (size_t)P->npass;
+        unsigned fl = 0;
+        const void *wk = line_table(prec, nl, L + 1, c->st);
+        int unit = pb->dims[r - 1].is == 1 && ((uintptr_t)pb->in % (2 * rs)) == 0;
+        for (i = 0; i < nb; ++i) unit &= (b[i].is % 2) == 0;
+        if (unit) fl |= FK_REAL_UNIT;
+        if (!wk) { c->err = 1; return -1; }
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, FK_BUF_IN, 0,
+                      FK_BUF_OUT, 0, fl, rs, cs);
+        for (; before < (size_t)P->npass; ++before) {
+            fk_pass *p = P->pass + before;
+            p->a.aux = wk;
+            p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+        }
+        if (unit) P->align_need = (unsigned)(2 * rs);
+        desc_add(P, "(r2c-fused L=%lld)", L);
+    } else {
+        /* widen -> full complex DFT along the last dim -> copy n/2+1 bins */
+        long long H = batch_count(nb, b), s;
+        fk_dim sb[FK_MAXRANK * 2];
+        fk_pass proto;
+        need_scratch(c, 6, cs * (size_t)nl * (size_t)H);
+        need_scratch(c, 7, cs * (size_t)nl * (size_t)H);
+        s = nl;
+        for (i = 0; i < nb; ++i) {
+            sb[i].n = b[i].n;
+            sb[i].is = b[i].is;
+            sb[i].os = s;
+            s *= b[i].n;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_WIDEN;
+        proto.e.n = nl;
+        proto.e.is = pb->dims[r - 1].is;
+        proto.e.os = 1;
+        proto.ibuf = FK_BUF_IN;
+        proto.obuf = FK_BUF_S0 + 6;
+        proto.bytes = (double)H * nl * (rs + cs);
+        emit_loops(c, &proto, nb, sb, rs, cs);
+        for (i = 0; i < nb; ++i) sb[i].is = sb[i].os;
+        if (plan_c2c_1d(c, nl, 1, 1, nb, sb, FK_BUF_S0 + 6, 0, FK_BUF_S0 + 7, 0, 0, 0, 0))
+            return -1;
+        for (i = 0; i < nb; ++i) sb[i].os = b[i].os;
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_COPY;
+        proto.e.n = nh;
+        proto.e.is = 1;
+        proto.e.os = pb->dims[r - 1].os;
+        proto.ibuf = FK_BUF_S0 + 7;
+        proto.obuf = FK_BUF_OUT;
+        proto.bytes = (double)H * nh * 2 * cs;
+        emit_loops(c, &proto, nb, sb, cs, cs);
+        desc_add(P, "(r2c-generic n=%lld)", nl);
+    }
+    if (c->err) return -1;
+    /* leading dims: c2c forward in place on the output, nh columns */
+    if (r > 1) {
+        fk_dim vd[FK_MAXRANK * 2];
+        int nv = 0;
+        for (i = 0; i < r - 1; ++i) {
+            lead[i].n = pb->dims[i].n;
+            lead[i].is = lead[i].os = pb->dims[i].os;
+        }
+        vd[nv].n = nh;
+        vd[nv].is = vd[nv].os = pb->dims[r - 1].os;
+        ++nv;
+        for (i = 0; i < pb->vrank; ++i) {
+            vd[nv].n = pb->vdims[i].n;
+            vd[nv].is = vd[nv].os = pb->vdims[i].os;
+            ++nv;
+        }
+        if (plan_c2c_dims(c, r - 1, lead, nv, vd, FK_BUF_OUT, FK_BUF_OUT, 0)) return -1;
+    }
+    return 0;
+}
+
+static int plan_c2r(ctx *c, const fk_problem *pb)
+{
+    fk_plan *P = c->P;
+    int prec = P->prec, r = pb->rank, i;
+    size_t cs = csz(prec), rs = rsz(prec);
+    long long nl = pb->dims[r - 1].n, nh = nl / 2 + 1, L = nl / 2;
+    fk_dim b[FK_MAXRANK * 2], lead[FK_MAXRANK];
+    int nb = 0;
+    const fk_kernel *k = 0;
+    /* leading dims first: backward c2c in place on the input (destroys it,
+       api/plan-many-dft-c2r.c:41-42) */
+    if (r > 1) {
+        fk_dim vd[FK_MAXRANK * 2];
+        int nv = 0;
+        for (i = 0; i < r - 1; ++i) {
+            lead[i].n = pb->dims[i].n;
+            lead[i].is = lead[i].os = pb->dims[i].is;
+        }
+        vd[nv].n = nh;
+        vd[nv].is = vd[nv].os = pb->dims[r - 1].is;
+        ++nv;
+        for (i = 0; i < pb->vrank; ++i) {
+            vd[nv].n = pb->vdims[i].n;
+            vd[nv].is = vd[nv].os = pb->vdims[i].is;
+            ++nv;
+        }
+        if (plan_c2c_dims(c, r - 1, lead, nv, vd, FK_BUF_IN, FK_BUF_IN, 1)) return -1;
+    }
+    for (i = 0; i < r - 1; ++i) b[nb++] = pb->dims[i];
+    for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
+    nb = squeeze(nb, b);
+    sort_by_is(nb, b);
+    if (nl % 2 == 0) k = find_kernel(prec, L, FK_ROW, FK_IO_C2R);
+    if (k) {
+        size_t before = (size_t)P->npass;
+        unsigned fl = 0;
+        const void *wk = line_table(prec, nl, L + 1, c->st);
+        int unit = pb->dims[r - 1].os == 1 && ((uintptr_t)pb->out % (2 * rs)) == 0;
+        for (i = 0; i < nb; ++i) unit &= (b[i].os % 2) == 0;
+        if (unit) fl |= FK_REAL_UNIT;
+        if (!wk) { c->err = 1; return -1; }
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, FK_BUF_IN, 0,
+                      FK_BUF_OUT, 0, fl, cs, rs);
+        for (; before < (size_t)P->npass; ++before) {
+            fk_pass *p = P->pass + before;
+            p->a.aux = wk;
+            p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+        }
+        if (unit) P->align_need = (unsigned)(2 * rs);
+        desc_add(P, "(c2r-fused L=%lld)", L);
+    } else {
+        /* Hermitian expansion -> full complex inverse -> real part */
+        long long H = batch_count(nb, b), s;
+        fk_dim sb[FK_MAXRANK * 2];
+        fk_pass proto;
+        need_scratch(c, 6, cs * (size_t)nl * (size_t)H);
+        need_scratch(c, 7, cs * (size_t)nl * (size_t)H);
+        s = nl;
+        for (i = 0; i < nb; ++i) {
+            sb[i].n = b[i].n;
+            sb[i].is = b[i].is;
+            sb[i].os = s;
+            s *= b[i].n;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_HERM;
+        proto.e.n = nl;
+        proto.e.p0 = nl;
+        proto.e.is = pb->dims[r - 1].is;
+        proto.e.os = 1;
+        proto.ibuf = FK_BUF_IN;
+        proto.obuf = FK_BUF_S0 + 6;
+        proto.bytes = (double)H * (nh + nl) * cs;
+        emit_loops(c, &proto, nb, sb, cs, cs);
+        for (i = 0; i < nb; ++i) sb[i].is = sb[i].os;
+        if (plan_c2c_1d(c, nl, 1, 1, nb, sb, FK_BUF_S0 + 6, 0, FK_BUF_S0 + 7, 0, 1, 1, 0))
+            return -1;
+        for (i = 0; i < nb; ++i) sb[i].os = b[i].os;
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_REAL;
+        proto.e.n = nl;
+        proto.e.is = 1;
+        proto.e.os = pb->dims[r - 1].os;
+        proto.ibuf = FK_BUF_S0 + 7;
+        proto.obuf = FK_BUF_OUT;
+        proto.bytes = (double)H * nl * (cs + rs);
+        emit_loops(c, &proto, nb, sb, cs, rs);
+        desc_add(P, "(c2r-generic n=%lld)", nl);
+    }
+    return c->err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* validation and normalisation                                        */
+
+static int dims_kosher(const fk_problem *pb)
+{
+    int i;
+    if (pb->rank < 0 || pb->rank > FK_MAXRANK || pb->vrank < 0 || pb->vrank > FK_MAXRANK)
+        return 0;
+    for (i = 0; i < pb->rank; ++i)
+        if (pb->dims[i].n <= 0) return 0;
+    for (i = 0; i < pb->vrank; ++i)
+        if (pb->vdims[i].n < 0) return 0;
+    return 1;
+}
+
+static int inplace_ok(const fk_problem *pb)
+{
+    /* FFTW: in-place needs identical input and output locations
+       (dft/problem.c:95-99, kernel/tensor7.c:201-215) */
+    int i;
+    for (i = 0; i < pb->rank; ++i)
+        if (pb->dims[i].n > 1 && pb->dims[i].is != pb->dims[i].os) return 0;
+    for (i = 0; i < pb->vrank; ++i)
+        if (pb->vdims[i].n > 1 && pb->vdims[i].is != pb->vdims[i].os) return 0;
+    return 1;
+}
+
+static void count_flops(fk_plan *P)
+{
+    double N = 1, H = 1;
+    int i;
+    for (i = 0; i < P->rank; ++i) N *= (double)P->dims[i].n;
+    for (i = 0; i < P->vrank; ++i) H *= (double)P->vdims[i].n;
+    /* nominal 5 N log2 N (2.5 for real), split as in libbench2/mflops.c:20-27 */
+    {
+        double f = (N > 1 ? 5.0 * N * log2(N) : 0) * H;
+        if (P->kind != FK_C2C) f *= 0.5;
+        P->adds = 0.6 * f;
+        P->muls = 0.4 * f;
+        P->fmas = 0;
+    }
+}
+
+static fk_plan *plan_internal(const fk_problem *pb0, hipStream_t st)
+{
+    fk_problem pb = *pb0;
+    fk_plan *P;
+    ctx c;
+    int i, ok = 0;
+    if (fk_init()) return 0;
+    if (!dims_kosher(&pb)) {
+        set_err("invalid dimensions");
+        return 0;
+    }
+    P = (fk_plan *)calloc(1, sizeof(fk_plan));
+    P->kind = pb.kind;
+    P->prec = pb.prec;
+    P->sign = pb.sign;
+    P->flags = pb.flags;
+    P->rank = pb.rank;
+    P->vrank = pb.vrank;
+    memcpy(P->dims, pb.dims, sizeof(fk_dim) * pb.rank);
+    memcpy(P->vdims, pb.vdims, sizeof(fk_dim) * pb.vrank);
+    P->in = pb.in;
+    P->out = pb.out;
+    P->refcount = 1;
+    P->stream = g_stream;
+    P->align_need = (unsigned)(pb.kind == FK_C2C ? csz(pb.prec) : rsz(pb.prec));
+    c.P = P;
+    c.st = st;
+    c.err = 0;
+    count_flops(P);
+    for (i = 0; i < pb.vrank; ++i)
+        if (pb.vdims[i].n == 0) goto done_ok; /* howmany = 0: nothing to do */
+    if (pb.kind == FK_C2C) {
+        int sw = pb.sign != -1;
+        P->inplace = pb.in == pb.out;
+        if (P->inplace && !inplace_ok(&pb)) goto fail;
+        if (((uintptr_t)pb.in | (uintptr_t)pb.out) % csz(pb.prec)) {
+            set_err("arrays must be aligned to the complex element size");
+            goto fail;
+        }
+        if (pb.rank == 0) {
+            if (!P->inplace) plan_copy(&c, pb.vrank, pb.vdims, FK_EW_COPY, csz(pb.prec));
+            desc_add(P, "(dft-rank0-copy)");
+        } else {
+            desc_add(P, "(dft-%s rank=%d", sw ? "bwd" : "fwd", pb.rank);
+            for (i = 0; i < pb.rank; ++i) desc_add(P, " %lld", pb.dims[i].n);
+            desc_add(P, ")");
+            if (plan_c2c_dims(&c, pb.rank, pb.dims, pb.vrank, pb.vdims, FK_BUF_IN, FK_BUF_OUT,
+                              sw))
+                goto fail;
+        }
+    } else if (pb.kind == FK_R2C) {
+        P->inplace = pb.in == pb.out;
+        if (pb.rank == 0) {
+            /* rank-0 rdft2: x -> (x, 0) */
+            fk_pass proto;
+            fk_dim b[FK_MAXRANK];
+            int nb;
+            memcpy(b, pb.vdims, sizeof(fk_dim) * pb.vrank);
+            nb = squeeze(pb.vrank, b);
+            memset(&proto, 0, sizeof(proto));
+            proto.type = FK_PASS_EW;
+            proto.e.op = FK_EW_WIDEN;
+            proto.e.n = 1;
+            proto.ibuf = FK_BUF_IN;
+            proto.obuf = FK_BUF_OUT;
+            emit_loops(&c, &proto, nb, b, rsz(pb.prec), csz(pb.prec));
+        } else if (plan_r2c(&c, &pb)) {
+            goto fail;
+        }
+    } else {
+        P->inplace = pb.in == pb.out;
+        if (pb.rank > 1 && (pb.flags & FK_PRESERVE_INPUT)) {
+            /* no input-preserving multi-D c2r, as in FFTW */
+            set_err("multi-dimensional c2r cannot preserve its input");
+            goto fail;
+        }
+        if (pb.rank == 0) {
+            fk_pass proto;
+            fk_dim b[FK_MAXRANK];
+            int nb;
+            memcpy(b, pb.vdims, sizeof(fk_dim) * pb.vrank);
+            nb = squeeze(pb.vrank, b);
+            memset(&proto, 0, sizeof(proto));
+            proto.type = FK_PASS_EW;
+            proto.e.op = FK_EW_REAL;
+            proto.e.n = 1;
+            proto.ibuf = FK_BUF_IN;
+            proto.obuf = FK_BUF_OUT;
+            emit_loops(&c, &proto, nb, b, csz(pb.prec), rsz(pb.prec));
+        } else if (plan_c2r(&c, &pb)) {
+            goto fail;
+        }
+    }
+    if (c.err) goto fail;
+done_ok:
+    for (i = 0; i < P->nscratch; ++i) {
+        if (P->scratch_bytes[i]) {
+            P->scratch[i] = fk_malloc(P->scratch_bytes[i]);
+            if (!P->scratch[i]) {
+                set_err("scratch allocation of %zu bytes failed", P->scratch_bytes[i]);
+                goto fail;
+            }
+        }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        set_err("table generation failed");
+        goto fail;
+    }
+    ok = 1;
+fail:
+    if (!ok) {
+        fk_plan_destroy(P);
+        return 0;
+    }
+    return P;
+}
+
+fk_plan *fk_plan_create(const fk_problem *pb)
+{
+    fk_plan *p;
+    pthread_mutex_lock(&g_lock);
+    p = plan_internal(pb, g_stream);
+    pthread_mutex_unlock(&g_lock);
+    return p;
+}
+
+void fk_plan_destroy(fk_plan *p)
+{
+    int i;
+    if (!p) return;
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) fk_free(p->scratch[i]);
+    free(p->pass);
+    free(p);
+}
+
+void fk_plan_set_stream(fk_plan *p, hipStream_t s)
+{
+    if (p) p->stream = s;
+}
+
+/* ------------------------------------------------------------------ */
+/* execution                                                           */
+
+static int launch_pass(const fk_pass *ps, void *const *buf, hipStream_t st, int prec)
+{
+    if (ps->type == FK_PASS_STOCKHAM) {
+        fk_args a = ps->a;
+        void *args[1];
+        a.in = (const char *)buf[ps->ibuf] + ps->ioff;
+        a.out = (char *)buf[ps->obuf] + ps->ooff;
+        args[0] = &a;
+        return (int)hipLaunchKernel(ps->k->fn, (dim3){ps->grid, 1, 1}, (dim3){ps->block, 1, 1},
+                                    args, ps->lds, st);
+    } else {
+        fk_ew_args e = ps->e;
+        e.in = (const char *)buf[ps->ibuf] + ps->ioff;
+        e.out = (char *)buf[ps->obuf] + ps->ooff;
+        return fkm_ew(prec, &e, st);
+    }
+}
+
+int fk_execute(fk_plan *p, void *in, void *out)
+{
+    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    void *tmp[FK_MAX_SCRATCH];
+    int i, rc = 0, own = 0;
+    hipStream_t st = p->stream;
+    if (((uintptr_t)in | (uintptr_t)out) % p->align_need) {
+        set_err("execute: arrays not aligned as at planning (%u bytes)", p->align_need);
+        return (int)hipErrorInvalidValue;
+    }
+    buf[FK_BUF_IN] = in;
+    buf[FK_BUF_OUT] = out;
+    /* concurrent executes of one plan each get their own scratch */
+    if (__sync_bool_compare_and_swap(&p->scratch_busy, 0, 1)) {
+        own = 1;
+        for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
+    } else {
+        for (i = 0; i < FK_MAX_SCRATCH; ++i) {
+            tmp[i] = 0;
+            if (p->scratch_bytes[i] &&
+                hipMallocAsync(&tmp[i], p->scratch_bytes[i], st) != hipSuccess)
+                rc = (int)hipErrorOutOfMemory;
+            buf[FK_BUF_S0 + i] = tmp[i];
+        }
+    }
+    for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
+    if (!own) {
+        for (i = 0; i < FK_MAX_SCRATCH; ++i)
+            if (tmp[i]) (void)hipFreeAsync(tmp[i], st);
+    }
+    if (!rc && !g_async) rc = (int)hipStreamSynchronize(st);
+    if (own) __sync_lock_release(&p->scratch_busy);
+    if (rc) set_err("execute failed: %s", hipGetErrorString((hipError_t)rc));
+    return rc;
+}
+
+void fk_execute_or_die(fk_plan *p, void *in, void *out)
+{
+    if (fk_execute(p, in, out)) {
+        fprintf(stderr, "fftw3_amd: %s\n", fk_last_error());
+        abort();
+    }
+}
+
+/* Per-pass timing with HIP events on the plan's stream. */
+double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass_ms)
+{
+    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    hipEvent_t *ev;
+    double total = 0;
+    int i, it, rc = 0;
+    hipStream_t st = p->stream;
+    if (iters < 1) iters = 1;
+    buf[FK_BUF_IN] = in;
+    buf[FK_BUF_OUT] = out;
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
+    ev = (hipEvent_t *)calloc((size_t)p->npass + 1, sizeof(hipEvent_t));
+    for (i = 0; i <= p->npass; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) rc = -1;
+    for (i = 0; i < p->npass && per_pass_ms; ++i) per_pass_ms[i] = 0;
+    for (it = 0; it < iters && !rc; ++it) {
+        float ms;
+        rc = (int)hipEventRecord(ev[0], st);
+        for (i = 0; i < p->npass && !rc; ++i) {
+            rc = launch_pass(p->pass + i, buf, st, p->prec);
+            if (!rc) rc = (int)hipEventRecord(ev[i + 1], st);
+        }
+        if (!rc) rc = (int)hipEventSynchronize(ev[p->npass]);
+        for (i = 0; i < p->npass && !rc; ++i) {
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = -1;
+            if (per_pass_ms) per_pass_ms[i] += ms / iters;
+        }
+        if (!rc && hipEventElapsedTime(&ms, ev[0], ev[p->npass]) == hipSuccess) total += ms;
+    }
+    for (i = 0; i <= p->npass; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    free(ev);
+    if (rc) {
+        set_err("timing failed");
+        return -1;
+    }
+    return total / iters;
+}
+
+/* ------------------------------------------------------------------ */
+/* introspection                                                       */
+
+int fk_plan_npasses(const fk_plan *p) { return p ? p->npass : 0; }
+
+double fk_plan_bytes(const fk_plan *p)
+{
+    double b = 0;
+    int i;
+    for (i = 0; p && i < p->npass; ++i) b += p->pass[i].bytes;
+    return b;
+}
+
+static const char *io_name(int io)
+{
+    static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein"};
+    return io >= 0 && io < 5 ? n[io] : "?";
+}
+
+int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
+{
+    const fk_pass *ps;
+    if (!p || i < 0 || i >= p->npass) return -1;
+    ps = p->pass + i;
+    if (ps->type == FK_PASS_STOCKHAM) {
+        char rad[64] = "";
+        int j;
+        for (j = 0; j < ps->k->nrad; ++j)
+            snprintf(rad + strlen(rad), sizeof(rad) - strlen(rad), "%s%d", j ? "x" : "",
+                     ps->k->rad[j]);
+        return snprintf(buf, len,
+                        "stockham L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
+                        "is=%lld os=%lld flags=%u",
+                        ps->k->L, rad, ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
+                        ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
+                        ps->a.os, ps->a.flags);
+    }
+    return snprintf(buf, len, "elementwise op=%d n=%lld lines=%lld", ps->e.op, ps->e.n,
+                    ps->e.nlines);
+}
+
+char *fk_plan_sprint(const fk_plan *p)
+{
+    size_t cap = 4096, len = 0;
+    char *s = (char *)malloc(cap), line[512];
+    int i;
+    s[0] = 0;
+    if (!p) return s;
+    len = snprintf(s, cap, "(fftw3-amd-gfx950 %s %s\n", p->prec ? "float" : "double", p->desc);
+    for (i = 0; i < p->npass; ++i) {
+        size_t l;
+        fk_plan_pass_info(p, i, line, sizeof(line));
+        l = strlen(line) + 8;
+        if (len + l >= cap) {
+            cap = 2 * (len + l);
+            s = (char *)realloc(s, cap);
+        }
+        len += snprintf(s + len, cap - len, "  (%s)\n", line);
+    }
+    if (len + 4 >= cap) s = (char *)realloc(s, len + 4);
+    strcat(s, ")");
+    return s;
+}
+
+void fk_plan_print(const fk_plan *p, FILE *f)
+{
+    char *s = fk_plan_sprint(p);
+    fputs(s, f);
+    free(s);
+}
+
+/* ------------------------------------------------------------------ */
+
+void fk_cleanup(void)
+{
+    tbl *t, *n;
+    pthread_mutex_lock(&g_lock);
+    for (t = g_tbl; t; t = n) {
+        n = t->next;
+        fk_free(t->dev);
+        free(t);
+    }
+    g_tbl = 0;
+    pthread_mutex_unlock(&g_lock);
+}

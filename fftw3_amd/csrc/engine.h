@@ -1,0 +1,135 @@
+/*
+ * Host engine shared by the double (fftw_) and float (fftwf_) API
+ * instances.  A plan is a list of passes (kernel launches) over symbolic
+ * buffers {input, output, scratch slots}; execute binds the user's device
+ * pointers and launches the list on a HIP stream.
+ *
+ * Reference counterparts: problem/tensor types (dft/dft.h:34-38,
+ * rdft/rdft.h:117-124, kernel/ifftw.h:417-432), the apiplan
+ * (api/api.h:62-67) and the planner (kernel/planner.c:623-747), which here
+ * becomes a pass-list builder plus a GPU timing autotuner.
+ */
+#ifndef FK_ENGINE_H
+#define FK_ENGINE_H
+
+#include <stddef.h>
+#include <stdio.h>
+#include <hip/hip_runtime_api.h>
+#include "fk_args.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_MAXRANK 16
+#define FK_MAX_SCRATCH 8
+
+enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2 };
+enum { FK_BUF_IN = 0, FK_BUF_OUT = 1, FK_BUF_S0 = 2 };
+enum { FK_PASS_STOCKHAM = 0, FK_PASS_EW = 1 };
+
+/* planner flags, same values as fftw3.h (api/fftw3.h:495-519) */
+#define FK_MEASURE (0U)
+#define FK_DESTROY_INPUT (1U << 0)
+#define FK_UNALIGNED (1U << 1)
+#define FK_CONSERVE_MEMORY (1U << 2)
+#define FK_EXHAUSTIVE (1U << 3)
+#define FK_PRESERVE_INPUT (1U << 4)
+#define FK_PATIENT (1U << 5)
+#define FK_ESTIMATE (1U << 6)
+#define FK_WISDOM_ONLY (1U << 21)
+
+typedef struct { long long n, is, os; } fk_dim;
+
+typedef struct {
+    int type;
+    const fk_kernel *k;
+    fk_args a;
+    fk_ew_args e;
+    int ibuf, obuf;
+    long long ioff, ooff;       /* byte offsets from the bound buffers */
+    unsigned grid, block, lds;
+    double bytes;               /* HBM bytes the pass moves (estimate) */
+} fk_pass;
+
+typedef struct fk_plan {
+    int kind, prec, sign;
+    unsigned flags;
+    int rank, vrank;
+    fk_dim dims[FK_MAXRANK], vdims[FK_MAXRANK];
+    int inplace;
+    void *in, *out;             /* arrays given at planning */
+    int npass, cpass;
+    fk_pass *pass;
+    int nscratch;
+    size_t scratch_bytes[FK_MAX_SCRATCH];
+    void *scratch[FK_MAX_SCRATCH];
+    int scratch_busy;
+    unsigned align_need;        /* required alignment of in/out (bytes) */
+    hipStream_t stream;
+    int refcount;
+    double adds, muls, fmas;    /* nominal operation counts */
+    double pcost;               /* measured seconds per execute (MEASURE) */
+    char desc[2048];            /* plan description for print_plan */
+    int choice[8];              /* planner decisions (wisdom payload) */
+    int nchoice;
+} fk_plan;
+
+/* problem description handed to the planner (strides in elements of the
+   array they index, FFTW tensor semantics) */
+typedef struct {
+    int kind, prec, sign;
+    unsigned flags;
+    int rank, vrank;
+    fk_dim dims[FK_MAXRANK], vdims[FK_MAXRANK];
+    void *in, *out;
+} fk_problem;
+
+/* ---- engine entry points ---- */
+int fk_init(void);
+fk_plan *fk_plan_create(const fk_problem *pb);
+void fk_plan_destroy(fk_plan *p);
+int fk_execute(fk_plan *p, void *in, void *out);         /* 0 or hip error */
+void fk_execute_or_die(fk_plan *p, void *in, void *out);
+void fk_cleanup(void);
+void fk_forget_wisdom(void);
+void fk_set_timelimit(double t);
+
+/* device memory */
+void *fk_malloc(size_t n);
+void fk_free(void *p);
+
+/* wisdom (text) */
+char *fk_wisdom_export(void);                 /* malloc'd string */
+int fk_wisdom_import(const char *s);          /* 1 ok, 0 rejected */
+
+/* extensions */
+void fk_set_stream(hipStream_t s);
+hipStream_t fk_get_stream(void);
+void fk_set_async(int on);
+int fk_get_async(void);
+const char *fk_last_error(void);
+void fk_plan_set_stream(fk_plan *p, hipStream_t s);
+void fk_plan_print(const fk_plan *p, FILE *f);
+char *fk_plan_sprint(const fk_plan *p);
+int fk_plan_npasses(const fk_plan *p);
+int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len);
+double fk_plan_bytes(const fk_plan *p);
+double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass_ms);
+
+/* launchers from fk_misc.hip */
+int fkm_gen_stage_tw(void *out, int prec, int nrad, const int *rad, int total, hipStream_t st);
+int fkm_gen_tw2(double *lo, double *hi, long long N, long long b, hipStream_t st);
+int fkm_gen_line_tw(void *out, int prec, long long n, long long count, hipStream_t st);
+int fkm_gen_chirp(void *out, int prec, long long n, hipStream_t st);
+int fkm_gen_blue_b(void *out, long long n, long long M, hipStream_t st);
+int fkm_scale_round(void *out, int prec, const void *in, long long count, double scale,
+                    hipStream_t st);
+int fkm_fill(void *out, int prec, long long count, unsigned long long seed,
+             unsigned long long stream, int gauss, hipStream_t st);
+int fkm_ew(int prec, const fk_ew_args *a, hipStream_t st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

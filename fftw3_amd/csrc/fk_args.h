@@ -1,0 +1,105 @@
+/*
+ * Kernel argument block and kernel registry shared by the C host library
+ * and the HIP kernels.  Plain C types only: the same struct is filled by
+ * the host and passed by value to every transform kernel.
+ */
+#ifndef FK_ARGS_H
+#define FK_ARGS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* compute mapping of threads onto (line, butterfly) */
+enum { FK_ROW = 0,   /* threads of a line are consecutive: unit-stride lines */
+       FK_COL = 1 }; /* lines of a tile are consecutive: unit-stride batch */
+
+/* what the kernel does around the Stockham stages */
+enum {
+    FK_IO_PLAIN = 0,  /* load line, FFT, store line */
+    FK_IO_TOUT = 1,   /* store through LDS with lines-fastest mapping */
+    FK_IO_R2C = 2,    /* real input viewed as L complex; store n/2+1 bins */
+    FK_IO_C2R = 3,    /* load n/2+1 bins, half-length inverse, real output */
+    FK_IO_BLUE = 4    /* fused Bluestein convolution of length L */
+};
+
+/* flags */
+#define FK_SWAP_IN   1u   /* swap re/im on load  (backward transform) */
+#define FK_SWAP_OUT  2u   /* swap re/im on store (backward transform) */
+#define FK_TW_OUT    4u   /* multiply output bin k of line with batch index
+                             i0 by W_N^(k*i0*twm) (four-step twiddle) */
+#define FK_SCALE     8u   /* multiply output by scale */
+#define FK_REAL_UNIT 16u  /* R2C: real input stride is 1 / C2R: output stride 1 */
+
+typedef struct {
+    const void *in;
+    void *out;
+    long long is, os;         /* element strides within a line */
+    long long ids[3], ods[3]; /* batch strides, dim 0 fastest */
+    long long nb[3];          /* batch counts */
+    long long nlines;         /* nb[0]*nb[1]*nb[2] */
+    const void *tw;           /* per-stage twiddles, working precision */
+    /* four-step output twiddle W_N^e, e = (k*i0*twm) mod N, from an FP64
+       two-level table: W_N^e = tlo[e % tb] * thi[e / tb] */
+    const double *tlo, *thi;
+    long long tN, tb, twm;
+    /* FK_IO_R2C / FK_IO_C2R: W_n^k, k <= L (working precision)
+       FK_IO_BLUE: chirp c_m (m < nvalid), then FFT_L(conj chirp)/L */
+    const void *aux, *aux2;
+    long long nvalid;         /* BLUE: logical length n; else unused */
+    double scale;
+    unsigned flags;
+    unsigned xcd_remap;       /* nonzero: spread tiles XCD-contiguously */
+} fk_args;
+
+typedef struct {
+    int prec;          /* 0 = double, 1 = float */
+    int L;             /* FFT length computed by the kernel */
+    int nrad;
+    int rad[12];
+    int tpl;           /* threads per line */
+    int nl;            /* lines per workgroup */
+    int mode;          /* FK_ROW / FK_COL */
+    int io;            /* FK_IO_* */
+    unsigned lds;      /* dynamic LDS bytes per workgroup */
+    int twsz;          /* complex entries in the per-stage twiddle table */
+    const void *fn;    /* kernel host stub for hipLaunchKernel */
+} fk_kernel;
+
+/* elementwise passes around the Stockham kernels (multi-pass paths) */
+enum {
+    FK_EW_COPY = 0,     /* complex copy, optional swaps */
+    FK_EW_COPY_R,       /* real copy */
+    FK_EW_BS_PREP,      /* Bluestein: a_j = x_j c_j (j < p0), 0 up to p1 */
+    FK_EW_BS_MUL,       /* Bluestein: swap(A_j * B_j) */
+    FK_EW_BS_POST,      /* Bluestein: y_k = swap(v_k) c_k, k < p0 */
+    FK_EW_R2C_POST,     /* Z (L = p0 complex) -> X[k], k <= L */
+    FK_EW_C2R_PRE,      /* X[k], k <= L -> Z (L = p0 complex) */
+    FK_EW_C2R_OUT,      /* z (p0 complex) -> interleaved real output */
+    FK_EW_WIDEN,        /* real -> complex (re, 0) */
+    FK_EW_HERM,         /* n/2+1 bins -> full Hermitian line of n = p0 */
+    FK_EW_REAL          /* complex -> real part */
+};
+
+typedef struct {
+    const void *in;
+    void *out;
+    long long n;              /* elements per line */
+    long long is, os;
+    long long ids[3], ods[3];
+    long long nb[3];
+    long long nlines;
+    const void *aux, *aux2;
+    long long p0, p1;
+    double scale;
+    unsigned flags;           /* FK_SWAP_IN / FK_SWAP_OUT / FK_SCALE */
+    int op;
+} fk_ew_args;
+
+/* registry, defined by the generated instantiation files */
+extern const fk_kernel *fk_registry(int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

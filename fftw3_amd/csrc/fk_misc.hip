@@ -1,0 +1,354 @@
+/*
+ * Support kernels and their C-ABI launchers:
+ *   - twiddle, chirp and Bluestein-kernel tables, generated in FP64 on the
+ *     GPU and rounded to the working precision (the role of
+ *     kernel/twiddle.c:124-256 and dft/bluestein.c:37-80);
+ *   - elementwise passes used between Stockham passes on the multi-pass,
+ *     Bluestein and odd-length real paths;
+ *   - counter-based input generation (splitmix64), identical to the oracle's.
+ */
+#include "fk_device.h"
+
+using namespace fk;
+
+namespace {
+
+struct stw_desc {
+    int nrad;
+    int rad[12];
+};
+
+template <typename R>
+__global__ void k_gen_stage_tw(cx<R> *out, stw_desc d, int total)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    /* locate stage s >= 1 owning entry t: stage s holds Ns*(Rs-1) entries
+       laid out [k][r-1] */
+    int off = 0, ns = d.rad[0];
+    for (int s = 1; s < d.nrad; ++s) {
+        int rs = d.rad[s], cnt = ns * (rs - 1);
+        if (t < off + cnt) {
+            int e = t - off, k = e / (rs - 1), r = e % (rs - 1) + 1;
+            double c, sn;
+            cexp_fp64((long long)r * k, (long long)ns * rs, c, sn);
+            out[t] = mk((R)c, (R)sn);
+            return;
+        }
+        off += cnt;
+        ns *= rs;
+    }
+}
+
+__global__ void k_gen_tw2(double *lo, double *hi, long long N, long long b)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long nhi = N / b + 1;
+    double c, s;
+    if (t < b) {
+        cexp_fp64(t, N, c, s);
+        lo[2 * t] = c;
+        lo[2 * t + 1] = s;
+    }
+    if (t < nhi) {
+        cexp_fp64(t * b, N, c, s);
+        hi[2 * t] = c;
+        hi[2 * t + 1] = s;
+    }
+}
+
+template <typename R>
+__global__ void k_gen_line_tw(cx<R> *out, long long n, long long count)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double c, s;
+    cexp_fp64(t, n, c, s);
+    out[t] = mk((R)c, (R)s);
+}
+
+/* c_m = exp(-i pi m^2 / n) = W_2n^(m^2 mod 2n) */
+template <typename R>
+__global__ void k_gen_chirp(cx<R> *out, long long n)
+{
+    long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    long long e = (long long)(((unsigned __int128)m * (unsigned long long)m) % (unsigned long long)(2 * n));
+    double c, s;
+    cexp_fp64(e, 2 * n, c, s);
+    out[m] = mk((R)c, (R)s);
+}
+
+/* b_m = conj(c_m) for |m| < n on a length-M circle, zero elsewhere (FP64) */
+__global__ void k_gen_blue_b(cx<double> *out, long long n, long long M)
+{
+    long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    long long m = -1;
+    if (j < n) m = j;
+    else if (M - j < n) m = M - j;
+    if (m < 0) { out[j] = mk(0.0, 0.0); return; }
+    long long e = (long long)(((unsigned __int128)m * (unsigned long long)m) % (unsigned long long)(2 * n));
+    double c, s;
+    cexp_fp64(e, 2 * n, c, s);
+    out[j] = mk(c, -s);
+}
+
+template <typename R>
+__global__ void k_scale_round(cx<R> *out, const cx<double> *in, long long count, double scale)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    out[t] = mk((R)(in[t].x * scale), (R)(in[t].y * scale));
+}
+
+__device__ __forceinline__ unsigned long long smix(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u01(unsigned long long seed, unsigned long long stream,
+                                      unsigned long long i)
+{
+    return (double)(smix(seed ^ (stream << 40) ^ i) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename R>
+__global__ void k_fill_uniform(R *out, long long count, unsigned long long seed,
+                               unsigned long long stream)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long step = (long long)gridDim.x * blockDim.x;
+    for (; t < count; t += step) out[t] = (R)(2.0 * u01(seed, stream, t) - 1.0);
+}
+
+template <typename R>
+__global__ void k_fill_gauss(R *out, long long count, unsigned long long seed,
+                             unsigned long long stream)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long step = (long long)gridDim.x * blockDim.x;
+    for (long long i = 2 * t; i < count; i += 2 * step) {
+        double a = u01(seed, stream, i), b = u01(seed, stream, i + 1);
+        double r = sqrt(-2.0 * log(1.0 - a));
+        double sn, cs;
+        sincospi(2.0 * b, &sn, &cs);
+        out[i] = (R)(r * cs);
+        if (i + 1 < count) out[i + 1] = (R)(r * sn);
+    }
+}
+
+template <typename R>
+__global__ void k_ew(fk_ew_args a)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = a.nlines * a.n;
+    if (t >= total) return;
+    const long long j = t % a.n;
+    long long q = t / a.n;
+    const long long i0 = q % a.nb[0];
+    q /= a.nb[0];
+    const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
+    const long long bi = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
+    const long long bo = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
+    const bool swi = a.flags & FK_SWAP_IN, swo = a.flags & FK_SWAP_OUT;
+    const cx<R> *ci = reinterpret_cast<const cx<R> *>(a.in) + bi;
+    cx<R> *co = reinterpret_cast<cx<R> *>(a.out) + bo;
+    const R *ri = reinterpret_cast<const R *>(a.in) + bi;
+    R *ro = reinterpret_cast<R *>(a.out) + bo;
+    const cx<R> *aux = reinterpret_cast<const cx<R> *>(a.aux);
+    const cx<R> *aux2 = reinterpret_cast<const cx<R> *>(a.aux2);
+    auto fin = [&](cx<R> v) {
+        if (a.flags & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        return swo ? swp(v) : v;
+    };
+    switch (a.op) {
+    case FK_EW_COPY: {
+        cx<R> v = ci[j * a.is];
+        if (swi) v = swp(v);
+        co[j * a.os] = fin(v);
+        break;
+    }
+    case FK_EW_COPY_R:
+        ro[j * a.os] = ri[j * a.is];
+        break;
+    case FK_EW_BS_PREP: {
+        cx<R> v = mk((R)0, (R)0);
+        if (j < a.p0) {
+            v = ci[j * a.is];
+            if (swi) v = swp(v);
+            v = mul(v, aux[j]);
+        }
+        co[j * a.os] = v;
+        break;
+    }
+    case FK_EW_BS_MUL:
+        co[j * a.os] = swp(mul(ci[j * a.is], aux2[j]));
+        break;
+    case FK_EW_BS_POST:
+        if (j < a.p0) co[j * a.os] = fin(mul(swp(ci[j * a.is]), aux[j]));
+        break;
+    case FK_EW_R2C_POST: {
+        const long long L = a.p0;
+        const cx<R> zk = ci[(j == L ? 0 : j) * a.is];
+        const cx<R> zc = ci[(j == 0 ? 0 : L - j) * a.is];
+        const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
+        const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
+        const cx<R> w = aux[j];
+        co[j * a.os] = fin(mk(er + orr * w.x - oi * w.y, ei + orr * w.y + oi * w.x));
+        break;
+    }
+    case FK_EW_C2R_PRE: {
+        const long long L = a.p0;
+        cx<R> xk = ci[j * a.is], xc = ci[(L - j) * a.is];
+        if (j == 0) { xk.y = 0; xc.y = 0; }
+        const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
+        const cx<R> d = mk(xk.x - xc.x, xk.y + xc.y);
+        const cx<R> w = aux[j];
+        const cx<R> o = mk(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+        co[j * a.os] = mk(e.x - o.y, e.y + o.x);
+        break;
+    }
+    case FK_EW_C2R_OUT: {
+        cx<R> v = ci[j * a.is];
+        if (a.flags & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        ro[(2 * j) * a.os] = v.x;
+        ro[(2 * j + 1) * a.os] = v.y;
+        break;
+    }
+    case FK_EW_WIDEN:
+        co[j * a.os] = mk(ri[j * a.is], (R)0);
+        break;
+    case FK_EW_HERM: {
+        const long long n = a.p0, h = n / 2;
+        cx<R> v;
+        if (j <= h) {
+            v = ci[j * a.is];
+            if (j == 0 || 2 * j == n) v.y = 0;
+        } else {
+            v = ci[(n - j) * a.is];
+            v.y = -v.y;
+        }
+        co[j * a.os] = v;
+        break;
+    }
+    case FK_EW_REAL: {
+        R v = ci[j * a.is].x;
+        if (a.flags & FK_SCALE) v *= (R)a.scale;
+        ro[j * a.os] = v;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+} // namespace
+
+extern "C" {
+
+int fkm_gen_stage_tw(void *out, int prec, int nrad, const int *rad, int total, hipStream_t st)
+{
+    if (total <= 0) return 0;
+    stw_desc d;
+    d.nrad = nrad;
+    for (int i = 0; i < 12; ++i) d.rad[i] = i < nrad ? rad[i] : 1;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_gen_stage_tw<double>, dim3(nblocks(total, 256)), dim3(256), 0, st,
+                           (cx<double> *)out, d, total);
+    else
+        hipLaunchKernelGGL(k_gen_stage_tw<float>, dim3(nblocks(total, 256)), dim3(256), 0, st,
+                           (cx<float> *)out, d, total);
+    return (int)hipGetLastError();
+}
+
+int fkm_gen_tw2(double *lo, double *hi, long long N, long long b, hipStream_t st)
+{
+    long long cnt = b > N / b + 1 ? b : N / b + 1;
+    hipLaunchKernelGGL(k_gen_tw2, dim3(nblocks(cnt, 256)), dim3(256), 0, st, lo, hi, N, b);
+    return (int)hipGetLastError();
+}
+
+int fkm_gen_line_tw(void *out, int prec, long long n, long long count, hipStream_t st)
+{
+    if (count <= 0) return 0;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_gen_line_tw<double>, dim3(nblocks(count, 256)), dim3(256), 0, st,
+                           (cx<double> *)out, n, count);
+    else
+        hipLaunchKernelGGL(k_gen_line_tw<float>, dim3(nblocks(count, 256)), dim3(256), 0, st,
+                           (cx<float> *)out, n, count);
+    return (int)hipGetLastError();
+}
+
+int fkm_gen_chirp(void *out, int prec, long long n, hipStream_t st)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL(k_gen_chirp<double>, dim3(nblocks(n, 256)), dim3(256), 0, st,
+                           (cx<double> *)out, n);
+    else
+        hipLaunchKernelGGL(k_gen_chirp<float>, dim3(nblocks(n, 256)), dim3(256), 0, st,
+                           (cx<float> *)out, n);
+    return (int)hipGetLastError();
+}
+
+int fkm_gen_blue_b(void *out, long long n, long long M, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gen_blue_b, dim3(nblocks(M, 256)), dim3(256), 0, st,
+                       (cx<double> *)out, n, M);
+    return (int)hipGetLastError();
+}
+
+int fkm_scale_round(void *out, int prec, const void *in, long long count, double scale,
+                    hipStream_t st)
+{
+    if (prec == 0)
+        hipLaunchKernelGGL(k_scale_round<double>, dim3(nblocks(count, 256)), dim3(256), 0, st,
+                           (cx<double> *)out, (const cx<double> *)in, count, scale);
+    else
+        hipLaunchKernelGGL(k_scale_round<float>, dim3(nblocks(count, 256)), dim3(256), 0, st,
+                           (cx<float> *)out, (const cx<double> *)in, count, scale);
+    return (int)hipGetLastError();
+}
+
+int fkm_fill(void *out, int prec, long long count, unsigned long long seed,
+             unsigned long long stream, int gauss, hipStream_t st)
+{
+    if (count <= 0) return 0;
+    unsigned nb = nblocks(count, 256);
+    if (nb > 65536) nb = 65536;
+    if (prec == 0) {
+        if (gauss)
+            hipLaunchKernelGGL(k_fill_gauss<double>, dim3(nb), dim3(256), 0, st, (double *)out,
+                               count, seed, stream);
+        else
+            hipLaunchKernelGGL(k_fill_uniform<double>, dim3(nb), dim3(256), 0, st, (double *)out,
+                               count, seed, stream);
+    } else {
+        if (gauss)
+            hipLaunchKernelGGL(k_fill_gauss<float>, dim3(nb), dim3(256), 0, st, (float *)out,
+                               count, seed, stream);
+        else
+            hipLaunchKernelGGL(k_fill_uniform<float>, dim3(nb), dim3(256), 0, st, (float *)out,
+                               count, seed, stream);
+    }
+    return (int)hipGetLastError();
+}
+
+int fkm_ew(int prec, const fk_ew_args *a, hipStream_t st)
+{
+    long long total = a->nlines * a->n;
+    if (total <= 0) return 0;
+    if (prec == 0)
+        hipLaunchKernelGGL(k_ew<double>, dim3(nblocks(total, 256)), dim3(256), 0, st, *a);
+    else
+        hipLaunchKernelGGL(k_ew<float>, dim3(nblocks(total, 256)), dim3(256), 0, st, *a);
+    return (int)hipGetLastError();
+}
+
+} // extern "C"

@@ -1,0 +1,321 @@
+/*
+ * LDS-resident Stockham kernel: every workgroup transforms NL lines of
+ * length L (compile-time L, radix sequence, thread layout).
+ *
+ * Stage s with radix Rs and Ns = R0*...*R(s-1) (autosort Stockham,
+ * decimation in time):
+ *   butterfly b in [0, L/Rs), k = b mod Ns
+ *   inputs   x[b + r*L/Rs]                        r = 0..Rs-1
+ *   twiddle  x_r *= W_{Ns*Rs}^{r*k}
+ *   outputs  y[(b/Ns)*Ns*Rs + k + r*Ns] = DFT_Rs(x)_r
+ * The first stage reads straight from HBM and the last stage writes straight
+ * to HBM (both patterns are unit-stride across threads); the stages between
+ * exchange through LDS.  This replaces FFTW's Cooley-Tukey recursion with
+ * twiddle codelets (dft/ct.c:34-45, dft/dftw-direct.c:46-56) for every
+ * length that fits one workgroup.
+ *
+ * Line addressing follows FFTW's tensor semantics (kernel/ifftw.h:417-432):
+ * element j of line (i0,i1,i2) is at i0*ids0 + i1*ids1 + i2*ids2 + j*is.
+ */
+#ifndef FK_STOCKHAM_H
+#define FK_STOCKHAM_H
+
+#include "fk_device.h"
+
+namespace fk {
+
+template <int... RS>
+struct RadList {
+    static constexpr int n = sizeof...(RS);
+    static constexpr int r[n] = {RS...};
+    static constexpr int L()
+    {
+        int p = 1;
+        for (int i = 0; i < n; ++i) p *= r[i];
+        return p;
+    }
+    static constexpr int ns(int s)
+    {
+        int p = 1;
+        for (int i = 0; i < s; ++i) p *= r[i];
+        return p;
+    }
+    /* offset of stage s in the twiddle table (stage 0 has none) */
+    static constexpr int twoff(int s)
+    {
+        int o = 0;
+        for (int t = 1; t < s; ++t) o += ns(t) * (r[t] - 1);
+        return o;
+    }
+    static constexpr int twsize() { return twoff(n); }
+};
+
+template <typename R, int L, int TPL, int NL, int MODE, int PAD>
+struct Geo {
+    /* LDS pitch of one line (ROW) — one pad slot every PAD elements, one
+       extra slot for the Nyquist bin of the real transforms */
+    static constexpr int LP = L + L / PAD + 2;
+    __device__ __forceinline__ static int idx(int lt, int j)
+    {
+        if constexpr (MODE == FK_ROW)
+            return lt * LP + j + j / PAD;
+        else
+            return j * NL + lt;
+    }
+    static constexpr unsigned lds_elems()
+    {
+        return MODE == FK_ROW ? (unsigned)(NL * LP) : (unsigned)((L + 2) * NL);
+    }
+};
+
+/*
+ * One Stockham stage.  LoadF(j) -> cx<R> and StoreF(k, cx<R>) abstract the
+ * data source and sink; IN_LDS/OUT_LDS say whether they touch LDS, which
+ * decides the barriers.
+ */
+template <typename R, class RL, int S, int L, int TPL, bool IN_LDS, bool OUT_LDS,
+          class LoadF, class StoreF>
+__device__ __forceinline__ void stage(int u, const cx<R> *__restrict__ tw,
+                                      LoadF &&ld, StoreF &&st)
+{
+    constexpr int Rs = RL::r[S];
+    constexpr int Ns = RL::ns(S);
+    constexpr int NB = L / Rs;
+    constexpr int BPT = (NB + TPL - 1) / TPL;
+    cx<R> v[BPT][Rs];
+    sfor<0, BPT>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int b = u + i * TPL;
+        if ((NB % TPL == 0) || b < NB) {
+            sfor<0, Rs>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                v[i][r] = ld(b + r * NB);
+            });
+        }
+    });
+    if constexpr (IN_LDS && OUT_LDS) __syncthreads();
+    sfor<0, BPT>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int b = u + i * TPL;
+        if ((NB % TPL == 0) || b < NB) {
+            const int k = b % Ns;
+            if constexpr (S > 0) {
+                const cx<R> *w = tw + RL::twoff(S) + k * (Rs - 1);
+                sfor<1, Rs>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    v[i][r] = mul(v[i][r], w[r - 1]);
+                });
+            }
+            Dft<R, Rs, 1>::run(v[i]);
+            const int base = (b - k) * Rs + k;
+            sfor<0, Rs>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                st(base + r * Ns, v[i][r]);
+            });
+        }
+    });
+    if constexpr (OUT_LDS) __syncthreads();
+}
+
+/* Runs stages [S, RL::n) ; stage S loads with `first`, the last stage stores
+   with `last`, everything between goes through LDS. */
+template <typename R, class RL, int S, int L, int TPL, bool FIRST_LDS,
+          bool LAST_LDS, class LdsF, class FirstF, class LastF>
+__device__ __forceinline__ void run_stages(int u, const cx<R> *tw, LdsF &&lds,
+                                           FirstF &&first, LastF &&last)
+{
+    constexpr int n = RL::n;
+    auto lds_ld = [&](int j) { return lds(j); };
+    auto lds_st = [&](int k, cx<R> v) { lds(k) = v; };
+    if constexpr (S == 0 && n == 1) {
+        stage<R, RL, 0, L, TPL, FIRST_LDS, LAST_LDS>(u, tw, first, last);
+    } else if constexpr (S == 0) {
+        stage<R, RL, 0, L, TPL, FIRST_LDS, true>(u, tw, first, lds_st);
+        run_stages<R, RL, 1, L, TPL, FIRST_LDS, LAST_LDS>(u, tw, lds, first, last);
+    } else if constexpr (S == n - 1) {
+        stage<R, RL, S, L, TPL, true, LAST_LDS>(u, tw, lds_ld, last);
+    } else {
+        stage<R, RL, S, L, TPL, true, true>(u, tw, lds_ld, lds_st);
+        run_stages<R, RL, S + 1, L, TPL, FIRST_LDS, LAST_LDS>(u, tw, lds, first, last);
+    }
+}
+
+__device__ __forceinline__ long long remap_tile(unsigned xcd)
+{
+    long long w = blockIdx.x;
+    if (!xcd) return w;
+    /* workgroups are dispatched round-robin over 8 XCDs; give each XCD a
+       contiguous run of tiles so neighbouring tiles share its L2 */
+    long long nwg = gridDim.x, per = nwg / 8, rem = nwg % 8;
+    long long x = w % 8, i = w / 8;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
+template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL>
+__global__ void __launch_bounds__(TPL * NL)
+stockham(fk_args a)
+{
+    constexpr int L = RL::L();
+    using G = Geo<R, L, TPL, NL, MODE, PAD>;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
+    long long *lbase = reinterpret_cast<long long *>(lds + G::lds_elems());
+
+    const int tid = threadIdx.x;
+    const int lt = (MODE == FK_ROW) ? tid / TPL : tid % NL;
+    const int u = (MODE == FK_ROW) ? tid % TPL : tid / NL;
+    const long long tile = remap_tile(a.xcd_remap);
+    const long long line = tile * NL + lt;
+    const bool act = line < a.nlines;
+
+    long long i0 = 0, i1 = 0, i2 = 0;
+    {
+        long long q = act ? line : 0;
+        i0 = q % a.nb[0];
+        q /= a.nb[0];
+        i1 = q % a.nb[1];
+        i2 = q / a.nb[1];
+    }
+    const long long bin = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
+    const long long bout = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
+    const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.tw);
+    const unsigned fl = a.flags;
+    const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
+
+    auto lds_at = [&](int j) -> cx<R> & { return lds[G::idx(lt, j)]; };
+
+    /* output post-op: twiddle, scale, swap */
+    auto post = [&](int k, cx<R> v) -> cx<R> {
+        if (fl & FK_TW_OUT) v = mul(v, tw2<R>(a, (long long)k * i0 * a.twm));
+        if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        return swo ? swp(v) : v;
+    };
+
+    if constexpr (IO == FK_IO_PLAIN || IO == FK_IO_TOUT) {
+        const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+        auto gld = [&](int j) -> cx<R> {
+            cx<R> v = act ? in[j * a.is] : mk((R)0, (R)0);
+            return swi ? swp(v) : v;
+        };
+        if constexpr (IO == FK_IO_PLAIN) {
+            auto gst = [&](int k, cx<R> v) {
+                if (act) out[bout + k * a.os] = post(k, v);
+            };
+            run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+        } else {
+            auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+            if (u == 0) lbase[lt] = act ? bout : -1;
+            run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+            /* lines-fastest store: consecutive threads write consecutive
+               lines of the tile (unit output batch stride) */
+            constexpr int NT = TPL * NL;
+            for (int e = tid; e < NL * L; e += NT) {
+                const int l2 = e % NL, k = e / NL;
+                const long long ob = lbase[l2];
+                if (ob >= 0) {
+                    cx<R> v = lds[G::idx(l2, k)];
+                    if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+                    out[ob + k * a.os] = swo ? swp(v) : v;
+                }
+            }
+        }
+    } else if constexpr (IO == FK_IO_R2C) {
+        /* real line of 2L samples viewed as L complex z_j = x_2j + i x_2j+1;
+           after the L-point DFT Z, X[k] = E[k] + W_2L^k O[k] with
+           E = (Z[k] + conj Z[L-k])/2, O = (Z[k] - conj Z[L-k])/(2i),
+           k = 0..L (the role of rdft/ct-hc2c.c:197-210's HC2C_VIA_DFT). */
+        const R *inr = reinterpret_cast<const R *>(a.in) + bin;
+        cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+        const cx<R> *wk = reinterpret_cast<const cx<R> *>(a.aux);
+        auto gld = [&](int j) -> cx<R> {
+            if (!act) return mk((R)0, (R)0);
+            if (fl & FK_REAL_UNIT) return reinterpret_cast<const cx<R> *>(inr)[j];
+            return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
+        };
+        auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+        run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+        for (int k = u; k <= L; k += TPL) {
+            const cx<R> zk = lds_at(k == L ? 0 : k);
+            const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
+            const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
+            /* O = (Z[k] - conj Z[L-k]) / (2i) */
+            const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
+            const cx<R> w = wk[k];
+            cx<R> X = mk(er + orr * w.x - oi * w.y, ei + orr * w.y + oi * w.x);
+            if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
+            if (act) out[bout + k * a.os] = X;
+        }
+    } else if constexpr (IO == FK_IO_C2R) {
+        /* half-length inverse: Z[k] = E[k] + i O[k],
+           E[k] = X[k] + conj X[L-k], O[k] = (X[k] - conj X[L-k]) W_2L^-k,
+           z = DFT^-1_L(Z) (unnormalized), y_2m = Re z_m, y_2m+1 = Im z_m.
+           Im X[0] and Im X[L] are ignored, as the r2cb codelets do. */
+        const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        R *outr = reinterpret_cast<R *>(a.out) + bout;
+        const cx<R> *wk = reinterpret_cast<const cx<R> *>(a.aux);
+        for (int k = u; k <= L; k += TPL) {
+            cx<R> v = act ? in[k * a.is] : mk((R)0, (R)0);
+            if (k == 0 || k == L) v.y = 0;
+            lds_at(k) = v;
+        }
+        __syncthreads();
+        auto pre = [&](int j) -> cx<R> {
+            const cx<R> xk = lds_at(j), xc = lds_at(L - j);
+            const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
+            const cx<R> d = mk(xk.x - xc.x, xk.y + xc.y);
+            const cx<R> w = wk[j];            /* W_2L^j; use its conjugate */
+            const cx<R> o = mk(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+            /* backward DFT through swapped forward: feed swap(Z) */
+            return mk(e.y + o.x, e.x - o.y);
+        };
+        auto gst = [&](int k, cx<R> v) {
+            /* v = swap(z_k) */
+            if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+            if (!act) return;
+            if (fl & FK_REAL_UNIT) {
+                reinterpret_cast<cx<R> *>(outr)[k] = mk(v.y, v.x);
+            } else {
+                outr[(2 * k) * a.os] = v.y;
+                outr[(2 * k + 1) * a.os] = v.x;
+            }
+        };
+        run_stages<R, RL, 0, L, TPL, true, false>(u, tw, lds_at, pre, gst);
+    } else if constexpr (IO == FK_IO_BLUE) {
+        /* Bluestein (dft/bluestein.c:82-128 conventions): a_j = x_j c_j,
+           c_j = exp(-i pi j^2/n); conv = IDFT_L(DFT_L(a) * B),
+           B = DFT_L(conj c)/L; y_k = c_k conv_k, k < n. */
+        const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+        const cx<R> *chirp = reinterpret_cast<const cx<R> *>(a.aux);
+        const cx<R> *Bt = reinterpret_cast<const cx<R> *>(a.aux2);
+        const int n = (int)a.nvalid;
+        auto gld = [&](int j) -> cx<R> {
+            if (!act || j >= n) return mk((R)0, (R)0);
+            cx<R> v = in[j * a.is];
+            if (swi) v = swp(v);
+            return mul(v, chirp[j]);
+        };
+        auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+        run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+        auto mid = [&](int j) -> cx<R> { return swp(mul(lds_at(j), Bt[j])); };
+        auto gst = [&](int k, cx<R> v) {
+            if (k < n && act) {
+                cx<R> y = mul(swp(v), chirp[k]);
+                if (fl & FK_SCALE) { y.x *= (R)a.scale; y.y *= (R)a.scale; }
+                out[bout + k * a.os] = swo ? swp(y) : y;
+            }
+        };
+        run_stages<R, RL, 0, L, TPL, true, false>(u, tw, lds_at, mid, gst);
+    }
+}
+
+template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL>
+constexpr unsigned stockham_lds()
+{
+    using G = Geo<R, RL::L(), TPL, NL, MODE, PAD>;
+    return G::lds_elems() * sizeof(cx<R>) + NL * sizeof(long long);
+}
+
+} // namespace fk
+#endif

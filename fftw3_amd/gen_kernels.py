@@ -1,0 +1,197 @@
+"""Generate the compile-time Stockham kernel catalog for gfx950.
+
+Writes fftw3_amd/csrc/gen/fk_inst_<i>.hip (template instantiations split for
+parallel compilation) and fftw3_amd/csrc/gen/fk_registry.hip (the table the
+host planner searches).  Each entry fixes: precision, length L, radix
+sequence, threads per line (TPL), lines per workgroup (NL), thread mapping
+(ROW / COL) and the IO kind (plain, transposed store, r2c, c2r, Bluestein).
+
+The catalog plays the role of FFTW's codelet lists
+(dft/scalar/codelets/Makefile.am:19-52, dft/simd/codlist.mk:15-78): it says
+which lengths run as one fused pass; every other length is decomposed by the
+host planner into several passes or a Bluestein convolution.
+"""
+import itertools
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "csrc", "gen")
+
+LDS_MAX = 160 * 1024          # per workgroup on gfx950
+LDS_TARGET = 64 * 1024        # keep >= 2 workgroups per CU when possible
+CSZ = {"double": 16, "float": 8}
+PREC = {"double": 0, "float": 1}
+RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
+
+
+def factorizations(n, maxr=16):
+    """All ordered-by-size multisets of radices from RADICES with product n."""
+    out = []
+
+    def rec(m, lo, acc):
+        if m == 1:
+            out.append(list(acc))
+            return
+        for r in RADICES:
+            if r > lo or m % r:
+                continue
+            rec(m // r, r, acc + [r])
+
+    rec(n, maxr, [])
+    return out
+
+
+def choose_radices(n):
+    """Fewest stages, then most balanced (largest smallest radix); larger
+    radices first (stage 0 reads HBM with stride L/R0)."""
+    fs = factorizations(n)
+    if not fs:
+        return None
+    fs.sort(key=lambda f: (len(f), -min(f), -sum(1 for r in f if r in (16, 8, 4, 2))))
+    return fs[0]
+
+
+def geometry(prec, L, rads, mode, io):
+    rmax = max(rads)
+    tpl = max(1, L // rmax)
+    pad = rads[0]
+    lp = L + L // pad + 2
+    csz = CSZ[prec]
+    if mode == "ROW":
+        per_line = lp * csz
+        nl = max(1, 256 // tpl)
+        while nl > 1 and nl * per_line > LDS_TARGET:
+            nl //= 2
+        if nl * per_line + 8 * nl > LDS_MAX:
+            return None
+    else:
+        per_line = (L + 2) * csz
+        want = 16 if prec == "float" else 8
+        nl = want
+        while nl > 1 and (nl * per_line > LDS_MAX - 1024 or nl * tpl > 1024):
+            nl //= 2
+        # small transforms: more lines per workgroup for occupancy
+        while nl * tpl < 256 and (nl * 2) * per_line <= LDS_TARGET and nl < 64:
+            nl *= 2
+    lds = (lp * nl if mode == "ROW" else (L + 2) * nl) * csz + 8 * nl
+    if lds > LDS_MAX or tpl * nl > 1024:
+        return None
+    return dict(tpl=tpl, nl=nl, pad=pad, lds=lds)
+
+
+def smooth(n, primes):
+    for p in primes:
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def size_list(prec):
+    lmax = 8192 if prec == "double" else 16384
+    s = set()
+    for n in range(2, 65):
+        if smooth(n, (2, 3, 5, 7, 11, 13)):
+            s.add(n)
+    k = 1
+    while k <= lmax:
+        for m in (1, 3, 5):
+            if 2 <= k * m <= lmax:
+                s.add(k * m)
+        k *= 2
+    for base in (3, 5, 7):
+        p = base
+        while p <= lmax:
+            s.add(p)
+            p *= base
+    s.update([100, 1000, 1001, 2187, 81, 729])
+    if lmax >= 10000:
+        s.add(10000)
+    return sorted(x for x in s if x <= lmax)
+
+
+def catalog():
+    entries = []
+    for prec in ("double", "float"):
+        lmax_plain = 8192 if prec == "double" else 16384
+        for L in size_list(prec):
+            rads = choose_radices(L)
+            if rads is None:
+                continue
+            fam = smooth(L, (2,)) or (L % 3 == 0 and smooth(L // 3, (2,))) or \
+                (L % 5 == 0 and smooth(L // 5, (2,)))
+            for mode, io in (("ROW", "PLAIN"), ("COL", "PLAIN"), ("ROW", "TOUT")):
+                if io == "TOUT" and (L < 16 or not fam):
+                    continue
+                g = geometry(prec, L, rads, mode, io)
+                if g:
+                    entries.append((prec, L, rads, mode, io, g))
+            # half-length kernels for even real transforms (n = 2L)
+            if (fam or L <= 64) and L <= 8192:
+                for io in ("R2C", "C2R"):
+                    g = geometry(prec, L, rads, "ROW", io)
+                    if g:
+                        entries.append((prec, L, rads, "ROW", io, g))
+            # fused Bluestein on power-of-two convolution lengths
+            if L & (L - 1) == 0 and L >= 4:
+                g = geometry(prec, L, rads, "ROW", "BLUE")
+                if g:
+                    entries.append((prec, L, rads, "ROW", "BLUE", g))
+    return entries
+
+
+def cxx_entry(e):
+    prec, L, rads, mode, io, g = e
+    rl = "fk::RadList<%s>" % ",".join(map(str, rads))
+    targs = "%s,%d,FK_%s,FK_IO_%s,%d,%s,%d" % (prec, g["nl"], mode, io, g["pad"], rl, g["tpl"])
+    return targs, rl
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    entries = catalog()
+    nparts = 8
+    parts = [[] for _ in range(nparts)]
+    # round-robin by estimated cost so parts compile in similar time
+    for i, e in enumerate(sorted(entries, key=lambda e: -e[1])):
+        parts[i % nparts].append(e)
+    written = set()
+    for p, es in enumerate(parts):
+        lines = ['// generated by fftw3_amd/gen_kernels.py -- do not edit',
+                 '#include "../fk_stockham.h"', '',
+                 'extern "C" const fk_kernel *fk_registry_part%d(int *n)' % p, '{',
+                 '    static const fk_kernel tab[] = {']
+        for e in es:
+            prec, L, rads, mode, io, g = e
+            targs, rl = cxx_entry(e)
+            radarr = ",".join(map(str, rads))
+            lines.append(
+                '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '
+                '%s::twsize(), (const void *)&fk::stockham<%s>},'
+                % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, targs, rl, targs))
+        lines += ['    };', '    *n = (int)(sizeof(tab) / sizeof(tab[0]));', '    return tab;', '}', '']
+        path = os.path.join(OUT, "fk_inst_%d.hip" % p)
+        written.add(path)
+        txt = "\n".join(lines)
+        if not os.path.exists(path) or open(path).read() != txt:
+            open(path, "w").write(txt)
+    reg = ['// generated by fftw3_amd/gen_kernels.py -- do not edit',
+           '#include <stdlib.h>', '#include <string.h>', '#include "../fk_args.h"', '']
+    for p in range(nparts):
+        reg.append('extern "C" const fk_kernel *fk_registry_part%d(int *n);' % p)
+    reg += ['', 'extern "C" const fk_kernel *fk_registry(int *count)', '{',
+            '    static fk_kernel *all = 0;', '    static int total = 0;',
+            '    if (!all) {', '        int n, off = 0;', '        const fk_kernel *t;',
+            '        fk_kernel *buf = (fk_kernel *)malloc(sizeof(fk_kernel) * %d);' % len(entries)]
+    for p in range(nparts):
+        reg.append('        t = fk_registry_part%d(&n); memcpy(buf + off, t, sizeof(fk_kernel) * n); off += n;' % p)
+    reg += ['        total = off;', '        all = buf;', '    }', '    *count = total;', '    return all;', '}', '']
+    path = os.path.join(OUT, "fk_registry.hip")
+    txt = "\n".join(reg)
+    if not os.path.exists(path) or open(path).read() != txt:
+        open(path, "w").write(txt)
+    print("kernels:", len(entries), file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+/*
+ * Project extensions of the MI355X engine.  These have no counterpart in
+ * the reference's fftw3.h: they exist because the drop-in's arrays live in
+ * device memory (copies, fills, streams) and for measurement.
+ *
+ * Plans of either precision (fftw_plan, fftwf_plan) are passed as void *.
+ */
+#ifndef FFTW3_AMD_H
+#define FFTW3_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Initialise the device runtime and kernel catalog; 0 on success.  Called
+   implicitly by every planner. */
+int fftw_amd_init(void);
+
+/* Last error message of this thread (planner returned NULL, execute failed). */
+const char *fftw_amd_last_error(void);
+
+/* Stream used by plans created from now on (hipStream_t, 0 = null stream),
+   and the stream of one existing plan. */
+void fftw_amd_set_stream(void *hip_stream);
+void *fftw_amd_get_stream(void);
+void fftw_amd_plan_set_stream(void *plan, void *hip_stream);
+
+/* 0 (default): execute returns after the transform completed, as FFTW's
+   execute does.  1: execute only enqueues work on the plan's stream. */
+void fftw_amd_set_async(int on);
+
+/* Device memory helpers; return 0 on success. */
+int fftw_amd_memcpy_htod(void *dst, const void *src, size_t bytes);
+int fftw_amd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int fftw_amd_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int fftw_amd_memset(void *dst, int value, size_t bytes);
+int fftw_amd_synchronize(void);
+size_t fftw_amd_mem_free(void);
+
+/* Counter-based synthetic inputs written on the device: element i of
+   `count` reals is 2u-1 with u the top 53 bits of
+   splitmix64(seed ^ (stream << 40) ^ i) / 2^53 (gauss: Box-Muller on
+   elements 2i, 2i+1).  prec: 0 double, 1 float. */
+int fftw_amd_fill_uniform(void *dev, int prec, size_t count, uint64_t seed, uint64_t stream);
+int fftw_amd_fill_gauss(void *dev, int prec, size_t count, uint64_t seed, uint64_t stream);
+
+/* Plan introspection */
+int fftw_amd_plan_npasses(const void *plan);
+int fftw_amd_plan_pass_info(const void *plan, int i, char *buf, int len);
+double fftw_amd_plan_bytes(const void *plan);
+/* Algorithmic HBM bytes of pass i (each input read once, output written once). */
+double fftw_amd_plan_pass_bytes(const void *plan, int i);
+
+/* Execute `plan` `iters` times on (in, out) with HIP events around every
+   pass; writes the mean milliseconds of pass i to per_pass_ms[i] (array of
+   fftw_amd_plan_npasses entries) and returns the mean milliseconds of a
+   whole execute.  Negative on error. */
+double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *per_pass_ms);
+
+/* Events on the current stream for timing regions (milliseconds). */
+void *fftw_amd_event_create(void);
+void fftw_amd_event_record(void *ev, void *hip_stream);
+double fftw_amd_event_elapsed(void *start, void *stop);
+void fftw_amd_event_destroy(void *ev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

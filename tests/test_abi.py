@@ -1,0 +1,74 @@
+"""CPU-side checks of the C-ABI boundary: the library loads, exports every
+entry point that include/*.h declares, and rejects invalid problems with a
+NULL plan before touching a device (api/plan-many-dft.c:35,
+api/mktensor-iodims.h:57-62)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    txt = open(os.path.join(ROOT, "include", "fftw3.h")).read()
+    for m in re.finditer(r"extern\s+[^;]*?P##(\w+)\s*\(", txt):
+        names.add("fftw_" + m.group(1))
+        names.add("fftwf_" + m.group(1))
+    for m in re.finditer(r"extern\s+const\s+char\s+P##(\w+)\[\]", txt):
+        names.add("fftw_" + m.group(1))
+        names.add("fftwf_" + m.group(1))
+    ext = open(os.path.join(ROOT, "include", "fftw3_amd.h")).read()
+    for m in re.finditer(r"^\w[\w\s\*]*?\b(fftw_amd_\w+)\s*\(", ext, re.M):
+        names.add(m.group(1))
+    return names
+
+
+def test_header_parses_to_expected_surface():
+    s = declared_symbols()
+    for must in ("fftw_plan_many_dft", "fftwf_plan_many_dft", "fftw_plan_guru_dft",
+                 "fftw_plan_guru64_dft", "fftw_plan_many_dft_r2c", "fftw_plan_many_dft_c2r",
+                 "fftw_execute_dft", "fftwf_execute_dft_r2c", "fftw_destroy_plan",
+                 "fftw_malloc", "fftw_version", "fftw_amd_fill_uniform"):
+        assert must in s, must
+    assert len(s) > 150
+
+
+def test_library_exports_every_declared_symbol():
+    so = os.path.join(ROOT, "fftw3_amd", "libfftw3_amd.so")
+    assert os.path.exists(so), "library not built"
+    out = subprocess.check_output(["nm", "-D", "--defined-only", so]).decode()
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = sorted(declared_symbols() - exported)
+    assert not missing, missing
+
+
+def test_loads_and_rejects_bad_problems_without_device():
+    import fftw3_amd as fa
+    L = fa.lib()
+    n = (ctypes.c_int * 2)(0, 4)
+    assert not L.fftw_plan_many_dft(1, n, 1, 0x1000, None, 1, 1, 0x2000, None, 1, 1, -1, fa.FFTW_ESTIMATE)
+    n[0] = 8
+    assert not L.fftwf_plan_many_dft(1, n, -3, 0x1000, None, 1, 1, 0x2000, None, 1, 1, -1, fa.FFTW_ESTIMATE)
+    assert not L.fftw_plan_many_dft(-1, n, 1, 0x1000, None, 1, 1, 0x2000, None, 1, 1, -1, fa.FFTW_ESTIMATE)
+    dims = (fa.IODim * 1)()
+    dims[0].n, dims[0].is_, dims[0].os = -2, 1, 1
+    assert not L.fftw_plan_guru_dft(1, dims, 0, dims, 0x1000, 0x2000, -1, fa.FFTW_ESTIMATE)
+    assert not L.fftw_plan_guru_split_dft(1, dims, 0, dims, 0x1000, 0x1008, 0x2000, 0x2008, 0)
+    assert L.fftw_alignment_of(0x1008) == 8
+    assert (ctypes.c_char * 16).in_dll(L, "fftw_version").value.startswith(b"fftw-3.3")
+
+
+def test_wisdom_string_roundtrip_contract():
+    import fftw3_amd as fa
+    L = fa.lib()
+    p = L.fftw_export_wisdom_to_string()
+    s = ctypes.cast(p, ctypes.c_char_p).value
+    L._libc.free(p)
+    assert s.startswith(b"(fftw3-amd-gfx950-wisdom")
+    assert L.fftw_import_wisdom_from_string(s) == 1
+    assert L.fftw_import_wisdom_from_string(b"(fftw-3.3.10 fftw_wisdom #x0 #x0 #x0 #x0)") == 0
+    assert L.fftw_import_wisdom_from_string(b"garbage") == 0
