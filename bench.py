@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--n2", type=int, default=2048)
     ap.add_argument("--howmany2", type=int, default=64)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 2 GiB rank-0 copy plan (known traffic) for counter calibration")
     return ap.parse_args()
 
 
@@ -106,6 +108,18 @@ def main():
     rt2 = oracle.rel_l2(yr / (n2 * n2), xr)
     log("round trip rel_l2: 3d fp64 %.3e, 2d fp32 %.3e" % (rt3, rt2))
 
+    if args.calib_copy:
+        # plain copy of 2^27 complex FP64 (2 GiB read + 2 GiB write): the
+        # reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters
+        cp = fa.plan_guru_dft("double", [], [(tot3, 1, 1)], b3.ptr, cmid.ptr if cmid.nbytes >= tot3 * 16
+                              else rout.ptr, fa.FFTW_FORWARD)
+        tmp = fa.DeviceBuffer(tot3 * 16)
+        cp.execute_dft(b3.ptr, tmp.ptr)
+        cp.execute_dft(b3.ptr, tmp.ptr)
+        L.fftw_amd_synchronize()
+        log("calibration copy: 2 dispatches of %d bytes read + %d bytes written" % (tot3 * 16, tot3 * 16))
+        del tmp
+
     def step():
         p3f.execute()
         p3b.execute()
@@ -147,6 +161,8 @@ def main():
             g[2] += 1
     key, (tms, tby, cnt) = max(groups.items(), key=lambda kv: kv[1][0])
     achieved = (tby / cnt) / (tms / cnt * 1e-3) / 1e9
+    symbol = kernel_symbol(key)
+    traffic = committed_traffic(symbol)
     log("dominant kernel: %s  launches/step=%d  mean %.3f ms  %.1f GB/s" % (key, cnt, tms / cnt, achieved))
     for k, (t, by, c) in sorted(groups.items(), key=lambda kv: -kv[1][0]):
         log("  %-90s %8.3f ms %7.1f GB/s x%d" % (k[:90], t, by / (t * 1e-3) / 1e9, c))
@@ -179,16 +195,45 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": key,
+            "kernel": symbol,
             "achieved": round(achieved, 1),
             "peak": PEAK_HBM,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": tby / cnt,
+            "mean_launch_ms": round(tms / cnt, 4),
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)
+
+
+def kernel_symbol(info):
+    """Demangled kernel name of a Stockham pass from its pass description
+    ("stockham f64 L=512 rad=8x8x8 col plain tpl=64 nl=8")."""
+    f = info.split()
+    if f[0] != "stockham":
+        return info
+    kv = dict(x.split("=", 1) for x in f if "=" in x)
+    rads = kv["rad"].split("x")
+    io = {"plain": 0, "tout": 1, "r2c": 2, "c2r": 3, "bluestein": 4}[f[5]]
+    return "void fk::stockham<%s, %s, %d, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
+        "double" if f[1] == "f64" else "float", kv["nl"], 0 if f[4] == "row" else 1, io, rads[0],
+        ", ".join(rads), kv["tpl"])
+
+
+def committed_traffic(symbol):
+    """HBM bytes per dispatch of `symbol` from the committed FETCH_SIZE /
+    WRITE_SIZE profile (profiles/traffic.json, calibrated on a copy kernel);
+    None when that kernel was not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        return t["kernels"].get(symbol)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(oracle, n3, n2):
