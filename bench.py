@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--n3", type=int, default=512)
     ap.add_argument("--n2", type=int, default=2048)
     ap.add_argument("--howmany2", type=int, default=64)
+    ap.add_argument("--estimate", action="store_true",
+                    help="plan with FFTW_ESTIMATE instead of FFTW_MEASURE (device autotuning)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 2 GiB rank-0 copy plan (known traffic) for counter calibration")
     return ap.parse_args()
@@ -77,14 +79,18 @@ def main():
     rout = fa.DeviceBuffer(H2 * n2 * n2 * 4)
 
     t0 = time.time()
+    pflags = fa.FFTW_ESTIMATE if args.estimate else fa.FFTW_MEASURE
     p3f = fa.plan_many_dft("double", [n3] * 3, 1, b3.ptr, None, 1, tot3, b3.ptr, None, 1, tot3,
-                           fa.FFTW_FORWARD, fa.FFTW_ESTIMATE)
+                           fa.FFTW_FORWARD, pflags)
     p3b = fa.plan_many_dft("double", [n3] * 3, 1, b3.ptr, None, 1, tot3, b3.ptr, None, 1, tot3,
-                           fa.FFTW_BACKWARD, fa.FFTW_ESTIMATE)
+                           fa.FFTW_BACKWARD, pflags)
     pr = fa.plan_many_dft_r2c("float", [n2, n2], H2, rin.ptr, None, 1, n2 * n2, cmid.ptr, None, 1,
-                              n2 * nh, fa.FFTW_ESTIMATE)
+                              n2 * nh, pflags)
     pc = fa.plan_many_dft_c2r("float", [n2, n2], H2, cmid.ptr, None, 1, n2 * nh, rout.ptr, None, 1,
-                              n2 * n2, fa.FFTW_ESTIMATE)
+                              n2 * n2, pflags)
+    # FFTW_MEASURE overwrote the arrays while timing candidates: refill
+    b3.fill_uniform("double", 2 * tot3, 2, 0)
+    rin.fill_uniform("float", H2 * n2 * n2, 2, 1)
     log("planned in %.2fs" % (time.time() - t0))
     plans = [("c2c3d_fwd", p3f), ("c2c3d_bwd", p3b), ("r2c2d", pr), ("c2r2d", pc)]
     for name, p in plans:
@@ -188,6 +194,7 @@ def main():
         "config": {
             "workload": "3-D c2c FP64 %d^3 in-place fwd+bwd + 2-D r2c/c2r FP32 %dx%d x%d" % (n3, n2, n2, H2),
             "parallelism": "single GPU",
+            "planner": "FFTW_ESTIMATE" if args.estimate else "FFTW_MEASURE (device autotune)",
             "gflops": round(gflops, 1),
             "bytes_per_step": bytes_step,
             "roundtrip_rel_l2": {"c2c3d_fp64": rt3, "r2c_c2r_2d_fp32": rt2},

@@ -24,6 +24,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "engine.h"
 
 /* ------------------------------------------------------------------ */
@@ -279,11 +280,64 @@ static int smooth_plannable(int prec, long long n, int depth)
 /* ------------------------------------------------------------------ */
 /* plan building                                                       */
 
+/* Planner decision kinds, ordered by the patience level that explores them
+   (MEASURE explores kernel geometries and thread mappings, PATIENT also
+   alternative two-pass splits). */
+enum { DEC_VARIANT = 1, DEC_MODE = 1, DEC_SPLIT = 2 };
+#define FK_MAX_DECIDE 32
+
 typedef struct {
     fk_plan *P;
     hipStream_t st;
     int err;
+    const int *choice;          /* decisions to replay (NULL: all defaults) */
+    int nchoice;
+    int ci;                     /* decisions taken so far */
+    int nalt[FK_MAX_DECIDE];    /* alternatives at each decision */
+    int dkind[FK_MAX_DECIDE];
 } ctx;
+
+static int decide(ctx *c, int nalt, int kind)
+{
+    int i = c->ci++, v = 0;
+    if (i < FK_MAX_DECIDE && c->choice && i < c->nchoice) v = c->choice[i];
+    if (v >= nalt) v = nalt - 1;
+    if (v < 0) v = 0;
+    if (i < FK_MAX_DECIDE) {
+        c->nalt[i] = nalt;
+        c->dkind[i] = kind;
+        c->P->choice[i] = v;
+    }
+    return v;
+}
+
+/* all catalog variants of (prec, L, mode, io), default first */
+static int find_variants(int prec, long long L, int mode, int io, const fk_kernel **out, int max)
+{
+    int i, n = 0, v;
+    if (L > (1 << 20)) return 0;
+    for (v = 0; v < max; ++v) {
+        const fk_kernel *hit = 0;
+        for (i = 0; i < g_nreg; ++i) {
+            const fk_kernel *k = g_reg + i;
+            if (k->prec == prec && k->L == L && k->mode == mode && k->io == io && k->variant == v) {
+                hit = k;
+                break;
+            }
+        }
+        if (!hit) break;
+        out[n++] = hit;
+    }
+    return n;
+}
+
+static const fk_kernel *pick_variant(ctx *c, int prec, long long L, int mode, int io)
+{
+    const fk_kernel *vs[16];
+    int n = find_variants(prec, L, mode, io, vs, 16);
+    if (!n) return 0;
+    return vs[n > 1 ? decide(c, n, DEC_VARIANT) : 0];
+}
 
 static fk_pass *new_pass(ctx *c)
 {
@@ -566,10 +620,15 @@ static int plan_bluestein(ctx *c, long long n, long long is, long long os, int n
 }
 
 /* pick n1 (column pass length) for a two-pass split of n */
-static long long pick_split(int prec, long long n)
+/* Candidate column lengths n1 for a two-pass split of n, best first:
+   balanced factors, a single-pass second factor, and column tiles of at
+   least 16 lines preferred.  Returns the number of candidates (<= max). */
+static int split_candidates(int prec, long long n, long long *out, int max)
 {
-    long long n1, best = 0;
-    double bestscore = 1e300;
+    long long n1;
+    double sc[64];
+    long long cand[64];
+    int nc = 0, i, j;
     for (n1 = 2; n1 <= 16384 && n1 < n; ++n1) {
         long long n2;
         double score;
@@ -580,12 +639,24 @@ static long long pick_split(int prec, long long n)
         score = fabs(log((double)n1) - log((double)n2));
         if (!has_single(prec, n2)) score += 100.0;
         if (n1 < 16) score += 10.0;
-        if (score < bestscore || (score == bestscore && n2 > n / best)) {
-            bestscore = score;
-            best = n1;
+        score -= 1e-9 * (double)n2; /* ties: larger second (row) factor */
+        if (nc < 64) {
+            sc[nc] = score;
+            cand[nc++] = n1;
         }
     }
-    return best;
+    for (i = 1; i < nc; ++i)
+        for (j = i; j > 0 && sc[j] < sc[j - 1]; --j) {
+            double t = sc[j];
+            long long u = cand[j];
+            sc[j] = sc[j - 1];
+            cand[j] = cand[j - 1];
+            sc[j - 1] = t;
+            cand[j - 1] = u;
+        }
+    if (nc > max) nc = max;
+    for (i = 0; i < nc; ++i) out[i] = cand[i];
+    return nc;
 }
 
 /*
@@ -630,50 +701,54 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
     }
     kr = find_kernel(prec, n, FK_ROW, FK_IO_PLAIN);
     if (kr) {
+        /* candidate thread mappings, the heuristic's choice first:
+           0 ROW, 1 ROW with transposed store, 2 COL */
+        int cand[3], ncand = 0, dT = -1, dC = -1, m;
+        long long best = 1LL << 62;
+        const fk_kernel *k;
         kc = find_kernel(prec, n, FK_COL, FK_IO_PLAIN);
         kt = find_kernel(prec, n, FK_ROW, FK_IO_TOUT);
-        if (llabs_(is) == 1) {
-            /* transposed store when the output line stride is large and some
-               batch dim is unit-stride on output */
-            int d = -1;
-            if (kt && llabs_(os) > 1)
-                for (i = 0; i < nb; ++i)
-                    if (llabs_(b[i].os) == 1) { d = i; break; }
-            if (d >= 0) {
-                fk_dim t = b[d];
-                memmove(b + 1, b, sizeof(fk_dim) * d);
-                b[0] = t;
-                emit_stockham(c, kt, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
-            } else {
-                emit_stockham(c, kr, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
-            }
-        } else {
-            int d = -1;
-            long long best = 1LL << 62;
-            for (i = 0; i < nb; ++i) {
-                long long s = llabs_(b[i].is) + llabs_(b[i].os);
-                if (s < best) { best = s; d = i; }
-            }
-            if (kc && d >= 0 && best <= 4 && b[d].n >= 2) {
-                fk_dim t = b[d];
-                memmove(b + 1, b, sizeof(fk_dim) * d);
-                b[0] = t;
-                emit_stockham(c, kc, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
-            } else {
-                emit_stockham(c, kr, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
-            }
+        if (kt && llabs_(os) > 1)
+            for (i = 0; i < nb; ++i)
+                if (llabs_(b[i].os) == 1) { dT = i; break; }
+        for (i = 0; i < nb; ++i) {
+            long long s = llabs_(b[i].is) + llabs_(b[i].os);
+            if (s < best && b[i].n >= 2) { best = s; dC = i; }
         }
+        if (!kc || best > 4) dC = -1;
+        if (llabs_(is) == 1) {
+            if (dT >= 0) cand[ncand++] = 1;
+            cand[ncand++] = 0;
+            if (dC >= 0) cand[ncand++] = 2;
+        } else {
+            if (dC >= 0) cand[ncand++] = 2;
+            cand[ncand++] = 0;
+            if (dT >= 0) cand[ncand++] = 1;
+        }
+        m = cand[ncand > 1 ? decide(c, ncand, DEC_MODE) : 0];
+        if (m == 1 || m == 2) {
+            int d = m == 1 ? dT : dC;
+            fk_dim t = b[d];
+            memmove(b + 1, b, sizeof(fk_dim) * d);
+            b[0] = t;
+        }
+        if (m == 1) k = pick_variant(c, prec, n, FK_ROW, FK_IO_TOUT);
+        else if (m == 2) k = pick_variant(c, prec, n, FK_COL, FK_IO_PLAIN);
+        else k = pick_variant(c, prec, n, FK_ROW, FK_IO_PLAIN);
+        emit_stockham(c, k, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
         return c->err ? -1 : 0;
     }
     if (smooth_plannable(prec, n, 0)) {
-        long long n1 = pick_split(prec, n), n2, H, s;
+        long long cands[4], n1, n2, H, s;
+        int ncand = split_candidates(prec, n, cands, 4);
         fk_dim ab[FK_MAXRANK * 2 + 1], bb[FK_MAXRANK * 2 + 1];
         fk_pass proto;
         const fk_kernel *kA;
         size_t before;
-        if (!n1) goto blue;
+        if (!ncand) goto blue;
+        n1 = cands[ncand > 1 ? decide(c, ncand, DEC_SPLIT) : 0];
         n2 = n / n1;
-        kA = find_kernel(prec, n1, FK_COL, FK_IO_PLAIN);
+        kA = pick_variant(c, prec, n1, FK_COL, FK_IO_PLAIN);
         H = batch_count(nb, b);
         need_scratch(c, slot, cs * (size_t)n * (size_t)H);
         /* pass A: lines n2_idx (tile dim), elements n1_idx at stride is*n2 */
@@ -798,7 +873,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
     for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
     nb = squeeze(nb, b);
     sort_by_is(nb, b);
-    if (nl % 2 == 0) k = find_kernel(prec, L, FK_ROW, FK_IO_R2C);
+    if (nl % 2 == 0) k = pick_variant(c, prec, L, FK_ROW, FK_IO_R2C);
     if (k) {
         size_t before = (size_t)P->npass;
         unsigned fl = 0;
@@ -910,7 +985,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
     for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
     nb = squeeze(nb, b);
     sort_by_is(nb, b);
-    if (nl % 2 == 0) k = find_kernel(prec, L, FK_ROW, FK_IO_C2R);
+    if (nl % 2 == 0) k = pick_variant(c, prec, L, FK_ROW, FK_IO_C2R);
     if (k) {
         size_t before = (size_t)P->npass;
         unsigned fl = 0;
@@ -1015,7 +1090,8 @@ static void count_flops(fk_plan *P)
     }
 }
 
-static fk_plan *plan_internal(const fk_problem *pb0, hipStream_t st)
+static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *choice, int nchoice,
+                           ctx *cout)
 {
     fk_problem pb = *pb0;
     fk_plan *P;
@@ -1040,9 +1116,12 @@ static fk_plan *plan_internal(const fk_problem *pb0, hipStream_t st)
     P->refcount = 1;
     P->stream = g_stream;
     P->align_need = (unsigned)(pb.kind == FK_C2C ? csz(pb.prec) : rsz(pb.prec));
+    memset(&c, 0, sizeof(c));
     c.P = P;
     c.st = st;
     c.err = 0;
+    c.choice = choice;
+    c.nchoice = nchoice;
     count_flops(P);
     for (i = 0; i < pb.vrank; ++i)
         if (pb.vdims[i].n == 0) goto done_ok; /* howmany = 0: nothing to do */
@@ -1123,12 +1202,133 @@ done_ok:
         set_err("table generation failed");
         goto fail;
     }
+    P->nchoice = c.ci < FK_MAX_DECIDE ? c.ci : FK_MAX_DECIDE;
+    if (cout) *cout = c;
     ok = 1;
 fail:
     if (!ok) {
         fk_plan_destroy(P);
         return 0;
     }
+    return P;
+}
+
+/* canonical problem key for wisdom (FFTW hashes the problem with md5,
+   kernel/planner.c:170-177) */
+static unsigned long long problem_hash(const fk_problem *pb)
+{
+    long long v[8 + 6 * FK_MAXRANK];
+    int m = 0, i;
+    v[m++] = pb->kind;
+    v[m++] = pb->prec;
+    v[m++] = pb->kind == FK_C2C ? pb->sign : 0;
+    v[m++] = pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT);
+    v[m++] = pb->in == pb->out;
+    v[m++] = (long long)((((uintptr_t)pb->in) | ((uintptr_t)pb->out)) % 16);
+    v[m++] = pb->rank;
+    v[m++] = pb->vrank;
+    for (i = 0; i < pb->rank; ++i) {
+        v[m++] = pb->dims[i].n;
+        v[m++] = pb->dims[i].is;
+        v[m++] = pb->dims[i].os;
+    }
+    for (i = 0; i < pb->vrank; ++i) {
+        v[m++] = pb->vdims[i].n;
+        v[m++] = pb->vdims[i].is;
+        v[m++] = pb->vdims[i].os;
+    }
+    return fk_fnv1a(v, sizeof(long long) * m, 0);
+}
+
+static int patience(unsigned flags)
+{
+    if (flags & FK_ESTIMATE) return 0;
+    if (flags & FK_EXHAUSTIVE) return 3;
+    if (flags & FK_PATIENT) return 2;
+    return 1; /* FFTW_MEASURE is 0: the default */
+}
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* mean ms of one execute on the planning arrays (FFTW_MEASURE may
+   overwrite them, doc/reference.texi:448-450) */
+static double time_plan(fk_plan *P, int reps)
+{
+    double t, best = 1e30;
+    int r;
+    if (fk_plan_time(P, P->in, P->out, 1, 0) < 0) return 1e30; /* warm-up */
+    for (r = 0; r < reps; ++r) {
+        t = fk_plan_time(P, P->in, P->out, 2, 0);
+        if (t < 0) return 1e30;
+        if (t < best) best = t;
+    }
+    return best;
+}
+
+/*
+ * Planner: wisdom lookup, else heuristic plan (ESTIMATE), else greedy
+ * search over the planner's decisions timed on the device (MEASURE: kernel
+ * geometries and thread mappings; PATIENT/EXHAUSTIVE: also the two-pass
+ * splits), the role of kernel/planner.c:518-615 and kernel/timer.c:142-181.
+ * The winning decisions become wisdom.
+ */
+static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
+{
+    int level = patience(pb->flags), wlevel = 0, wn = 0, i, a;
+    int wc[FK_MAX_DECIDE], cur[FK_MAX_DECIDE], ncur;
+    unsigned long long h = problem_hash(pb);
+    fk_plan *P, *Q;
+    ctx cb, cq;
+    double best, t0;
+    if (fk_wisdom_lookup(h, &wlevel, wc, &wn) &&
+        (wlevel >= level || (pb->flags & FK_WISDOM_ONLY))) {
+        P = build_plan(pb, st, wc, wn, 0);
+        if (P) P->tuned = 1;
+        return P;
+    }
+    if (pb->flags & FK_WISDOM_ONLY) {
+        set_err("FFTW_WISDOM_ONLY: no wisdom for this problem");
+        return 0;
+    }
+    P = build_plan(pb, st, 0, 0, &cb);
+    if (!P || level == 0 || !P->in || !P->out || P->npass == 0) return P;
+    t0 = now_s();
+    best = time_plan(P, level >= 3 ? 4 : 2);
+    ncur = P->nchoice;
+    memcpy(cur, P->choice, sizeof(int) * ncur);
+    for (i = 0; i < ncur && i < FK_MAX_DECIDE; ++i) {
+        if (cb.dkind[i] > level) continue;
+        for (a = 0; a < cb.nalt[i]; ++a) {
+            int trial[FK_MAX_DECIDE];
+            double t;
+            if (a == cur[i]) continue;
+            if (g_timelimit >= 0 && now_s() - t0 > g_timelimit) goto done;
+            memcpy(trial, cur, sizeof(int) * ncur);
+            trial[i] = a;
+            Q = build_plan(pb, st, trial, ncur, &cq);
+            if (!Q) continue;
+            t = time_plan(Q, level >= 3 ? 4 : 2);
+            if (t < best * 0.98) {
+                fk_plan_destroy(P);
+                P = Q;
+                best = t;
+                ncur = Q->nchoice;
+                memcpy(cur, Q->choice, sizeof(int) * ncur);
+                cb = cq;
+            } else {
+                fk_plan_destroy(Q);
+            }
+        }
+    }
+done:
+    P->pcost = best * 1e-3;
+    P->tuned = 2;
+    fk_wisdom_store(h, level, cur, ncur);
     return P;
 }
 

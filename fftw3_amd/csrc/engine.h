@@ -71,8 +71,9 @@ typedef struct fk_plan {
     double adds, muls, fmas;    /* nominal operation counts */
     double pcost;               /* measured seconds per execute (MEASURE) */
     char desc[2048];            /* plan description for print_plan */
-    int choice[8];              /* planner decisions (wisdom payload) */
+    int choice[32];             /* planner decisions (wisdom payload) */
     int nchoice;
+    int tuned;                  /* 0 heuristic, 1 from wisdom, 2 timed now */
 } fk_plan;
 
 /* problem description handed to the planner (strides in elements of the
@@ -102,6 +103,10 @@ void fk_free(void *p);
 /* wisdom (text) */
 char *fk_wisdom_export(void);                 /* malloc'd string */
 int fk_wisdom_import(const char *s);          /* 1 ok, 0 rejected */
+void fk_wisdom_store(unsigned long long h, int level, const int *c, int n);
+int fk_wisdom_lookup(unsigned long long h, int *level, int *c, int *n);
+int fk_wisdom_count(void);
+unsigned long long fk_fnv1a(const void *p, size_t n, unsigned long long h);
 
 /* extensions */
 void fk_set_stream(hipStream_t s);

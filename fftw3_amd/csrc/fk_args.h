@@ -64,6 +64,7 @@ typedef struct {
     unsigned lds;      /* dynamic LDS bytes per workgroup */
     int twsz;          /* complex entries in the per-stage twiddle table */
     const void *fn;    /* kernel host stub for hipLaunchKernel */
+    int variant;       /* 0 = default geometry; >0 autotuner alternatives */
 } fk_kernel;
 
 /* elementwise passes around the Stockham kernels (multi-pass paths) */

@@ -52,9 +52,12 @@ def choose_radices(n):
     return fs[0]
 
 
-def geometry(prec, L, rads, mode, io):
+def geometry(prec, L, rads, mode, io, tpl_div=1, nl_mul=1.0):
+    """Default geometry: one butterfly of the largest radix per thread and
+    stage (tpl = L / Rmax); tpl_div > 1 gives each thread tpl_div of them;
+    nl_mul scales the lines per workgroup of the default choice."""
     rmax = max(rads)
-    tpl = max(1, L // rmax)
+    tpl = max(1, L // rmax // tpl_div)
     pad = rads[0]
     lp = L + L // pad + 2
     csz = CSZ[prec]
@@ -74,6 +77,7 @@ def geometry(prec, L, rads, mode, io):
         # small transforms: more lines per workgroup for occupancy
         while nl * tpl < 256 and (nl * 2) * per_line <= LDS_TARGET and nl < 64:
             nl *= 2
+    nl = max(1, int(nl * nl_mul))
     lds = (lp * nl if mode == "ROW" else (L + 2) * nl) * csz + 8 * nl
     if lds > LDS_MAX or tpl * nl > 1024:
         return None
@@ -110,6 +114,29 @@ def size_list(prec):
     return sorted(x for x in s if x <= lmax)
 
 
+def variants(prec, L, rads, mode, io, g0):
+    """Alternative geometries for the autotuner on power-of-two lengths
+    64..8192: half the threads per line (two butterflies each), twice or half
+    the lines per workgroup, and the reversed radix order.  The default
+    geometry stays first in the registry (what FFTW_ESTIMATE uses)."""
+    if L & (L - 1) or L < 64 or L > 8192:
+        return []
+    out, seen = [], {(tuple(rads), g0["tpl"], g0["nl"])}
+    cands = [(rads, 2, 1.0), (rads, 1, 2.0), (rads, 1, 0.5)]
+    if list(reversed(rads)) != list(rads):
+        cands.append((list(reversed(rads)), 1, 1.0))
+    for r, td, nm in cands:
+        g = geometry(prec, L, r, mode, io, tpl_div=td, nl_mul=nm)
+        if not g or g["tpl"] < 16:
+            continue
+        key = (tuple(r), g["tpl"], g["nl"])
+        if key in seen:
+            continue
+        seen.add(key)
+        out.append((prec, L, list(r), mode, io, g))
+    return out
+
+
 def catalog():
     entries = []
     for prec in ("double", "float"):
@@ -126,12 +153,14 @@ def catalog():
                 g = geometry(prec, L, rads, mode, io)
                 if g:
                     entries.append((prec, L, rads, mode, io, g))
+                    entries += variants(prec, L, rads, mode, io, g)
             # half-length kernels for even real transforms (n = 2L)
             if (fam or L <= 64) and L <= 8192:
                 for io in ("R2C", "C2R"):
                     g = geometry(prec, L, rads, "ROW", io)
                     if g:
                         entries.append((prec, L, rads, "ROW", io, g))
+                        entries += variants(prec, L, rads, "ROW", io, g)
             # fused Bluestein on power-of-two convolution lengths
             if L & (L - 1) == 0 and L >= 4:
                 g = geometry(prec, L, rads, "ROW", "BLUE")
@@ -150,6 +179,15 @@ def cxx_entry(e):
 def main():
     os.makedirs(OUT, exist_ok=True)
     entries = catalog()
+    # variant index within (precision, L, mapping, io): 0 is the default
+    vcount = {}
+    numbered = []
+    for e in entries:
+        key = (e[0], e[1], e[3], e[4])
+        v = vcount.get(key, 0)
+        vcount[key] = v + 1
+        numbered.append(e + (v,))
+    entries = numbered
     nparts = 8
     parts = [[] for _ in range(nparts)]
     # round-robin by estimated cost so parts compile in similar time
@@ -162,13 +200,13 @@ def main():
                  'extern "C" const fk_kernel *fk_registry_part%d(int *n)' % p, '{',
                  '    static const fk_kernel tab[] = {']
         for e in es:
-            prec, L, rads, mode, io, g = e
-            targs, rl = cxx_entry(e)
+            prec, L, rads, mode, io, g, var = e
+            targs, rl = cxx_entry(e[:6])
             radarr = ",".join(map(str, rads))
             lines.append(
                 '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '
-                '%s::twsize(), (const void *)&fk::stockham<%s>},'
-                % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, targs, rl, targs))
+                '%s::twsize(), (const void *)&fk::stockham<%s>, %d},'
+                % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, targs, rl, targs, var))
         lines += ['    };', '    *n = (int)(sizeof(tab) / sizeof(tab[0]));', '    return tab;', '}', '']
         path = os.path.join(OUT, "fk_inst_%d.hip" % p)
         written.add(path)
