@@ -131,6 +131,16 @@ def lib():
     L.fftw_amd_plan_pass_bytes.restype = ctypes.c_double
     L.fftw_amd_plan_time.argtypes = [vp, vp, vp, ip, ctypes.POINTER(ctypes.c_double)]
     L.fftw_amd_plan_time.restype = ctypes.c_double
+    L.fftw_amd_plan_batch_dft_1d.restype = vp
+    L.fftw_amd_plan_batch_dft_1d.argtypes = [ip, ip, ipp, ipp, ctypes.POINTER(vp),
+                                             ctypes.POINTER(vp), ip, u]
+    for n_ in ("fftw_amd_batch_execute", "fftw_amd_batch_execute_direct"):
+        getattr(L, n_).argtypes = [vp]
+        getattr(L, n_).restype = ctypes.c_int
+    L.fftw_amd_batch_info.argtypes = [vp, ctypes.c_char_p, ip]
+    L.fftw_amd_batch_stream.argtypes = [vp]
+    L.fftw_amd_batch_stream.restype = vp
+    L.fftw_amd_batch_destroy.argtypes = [vp]
     L.fftw_amd_event_create.restype = vp
     L.fftw_amd_event_record.argtypes = [vp, vp]
     L.fftw_amd_event_elapsed.argtypes = [vp, vp]
@@ -271,6 +281,46 @@ class Plan:
     def destroy(self):
         if self.h:
             fn(self.prec, "destroy_plan")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """fftw_amd_plan_batch_dft_1d handle: groups of (n, howmany, in, out)."""
+
+    def __init__(self, prec, groups, sign=FFTW_FORWARD, flags=FFTW_ESTIMATE):
+        k = len(groups)
+        n = (ctypes.c_int * k)(*[g[0] for g in groups])
+        h = (ctypes.c_int * k)(*[g[1] for g in groups])
+        ins = (ctypes.c_void_p * k)(*[g[2] for g in groups])
+        outs = (ctypes.c_void_p * k)(*[g[3] for g in groups])
+        self.h = lib().fftw_amd_plan_batch_dft_1d(0 if prec == "double" else 1, k, n, h, ins, outs,
+                                                  sign, flags)
+        if not self.h:
+            raise RuntimeError("batch planner returned NULL: " + last_error())
+
+    def execute(self, graph=True):
+        f = lib().fftw_amd_batch_execute if graph else lib().fftw_amd_batch_execute_direct
+        r = f(self.h)
+        if r:
+            raise RuntimeError("batch execute failed (%d): %s" % (r, last_error()))
+
+    def info(self):
+        buf = ctypes.create_string_buffer(1024)
+        lib().fftw_amd_batch_info(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def stream(self):
+        return lib().fftw_amd_batch_stream(self.h)
+
+    def destroy(self):
+        if self.h:
+            lib().fftw_amd_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -684,6 +684,21 @@ double fftw_amd_plan_time(void *p, void *in, void *out, int iters, double *per_p
     return p ? fk_plan_time((fk_plan *)p, in, out, iters, per_pass_ms) : -1;
 }
 
+fftw_amd_batch fftw_amd_plan_batch_dft_1d(int prec, int ngroups, const int *n,
+                                          const int *howmany, void *const *in, void *const *out,
+                                          int sign, unsigned flags)
+{
+    return (fftw_amd_batch)fk_batch_create(prec, ngroups, n, howmany, in, out, sign, flags);
+}
+int fftw_amd_batch_execute(fftw_amd_batch b) { return fk_batch_execute((fk_batch *)b, 1); }
+int fftw_amd_batch_execute_direct(fftw_amd_batch b) { return fk_batch_execute((fk_batch *)b, 0); }
+int fftw_amd_batch_info(fftw_amd_batch b, char *buf, int len)
+{
+    return fk_batch_info((fk_batch *)b, buf, len);
+}
+void *fftw_amd_batch_stream(fftw_amd_batch b) { return (void *)fk_batch_stream((fk_batch *)b); }
+void fftw_amd_batch_destroy(fftw_amd_batch b) { fk_batch_destroy((fk_batch *)b); }
+
 void *fftw_amd_event_create(void)
 {
     hipEvent_t e = 0;

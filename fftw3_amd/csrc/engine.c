@@ -1412,6 +1412,19 @@ int fk_execute(fk_plan *p, void *in, void *out)
     return rc;
 }
 
+/* Enqueue every pass on `st` with the plan's own scratch; no host
+   synchronisation (usable inside stream capture). */
+int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st)
+{
+    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    int i, rc = 0;
+    buf[FK_BUF_IN] = in;
+    buf[FK_BUF_OUT] = out;
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
+    for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
+    return rc;
+}
+
 void fk_execute_or_die(fk_plan *p, void *in, void *out)
 {
     if (fk_execute(p, in, out)) {

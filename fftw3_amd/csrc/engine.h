@@ -122,6 +122,17 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len);
 double fk_plan_bytes(const fk_plan *p);
 double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass_ms);
 
+/* batched uneven sizes (batch.c) */
+typedef struct fk_batch fk_batch;
+fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howmany,
+                          void *const *in, void *const *out, int sign, unsigned flags);
+int fk_batch_execute(fk_batch *b, int use_graph);
+int fk_batch_info(fk_batch *b, char *buf, int len);
+hipStream_t fk_batch_stream(fk_batch *b);
+void fk_batch_destroy(fk_batch *b);
+int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st);
+int fkm_reset_counters(unsigned long long *c, int n, hipStream_t st);
+
 /* launchers from fk_misc.hip */
 int fkm_gen_stage_tw(void *out, int prec, int nrad, const int *rad, int total, hipStream_t st);
 int fkm_gen_tw2(double *lo, double *hi, long long N, long long b, hipStream_t st);

@@ -100,6 +100,28 @@ typedef struct {
 /* registry, defined by the generated instantiation files */
 extern const fk_kernel *fk_registry(int *count);
 
+/* One pass of a batched set executed by the persistent kernel: tiles
+   [first, first + tiles) of the global work counter run geometry `kind` of
+   the persistent menu on the lines described by `a`. */
+typedef struct {
+    fk_args a;
+    int kind;
+    int pad_;
+    long long tiles;
+    long long first;
+} fk_bpass;
+
+typedef struct {
+    int prec, L, nrad;
+    int rad[12];
+    int tpl, nl, mode, io;
+} fk_menu_entry;
+
+/* persistent-kernel menu of a precision: entries, kernel stub, dynamic LDS
+   bytes and workgroup size (generated) */
+extern const fk_menu_entry *fk_persist_menu(int prec, int *n, const void **fn, unsigned *lds,
+                                            int *nt);
+
 #ifdef __cplusplus
 }
 #endif

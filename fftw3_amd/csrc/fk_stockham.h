@@ -50,21 +50,24 @@ struct RadList {
     static constexpr int twsize() { return twoff(n); }
 };
 
-template <typename R, int L, int TPL, int NL, int MODE, int PAD>
+/* XL = 1 adds one scratch line for threads beyond TPL*NL (a persistent
+   workgroup running a geometry smaller than itself). */
+template <typename R, int L, int TPL, int NL, int MODE, int PAD, int XL = 0>
 struct Geo {
     /* LDS pitch of one line (ROW) — one pad slot every PAD elements, one
        extra slot for the Nyquist bin of the real transforms */
     static constexpr int LP = L + L / PAD + 2;
+    static constexpr int NLP = NL + XL;
     __device__ __forceinline__ static int idx(int lt, int j)
     {
         if constexpr (MODE == FK_ROW)
             return lt * LP + j + j / PAD;
         else
-            return j * NL + lt;
+            return j * NLP + lt;
     }
     static constexpr unsigned lds_elems()
     {
-        return MODE == FK_ROW ? (unsigned)(NL * LP) : (unsigned)((L + 2) * NL);
+        return MODE == FK_ROW ? (unsigned)(NLP * LP) : (unsigned)((L + 2) * NLP);
     }
 };
 
@@ -151,22 +154,31 @@ __device__ __forceinline__ long long remap_tile(unsigned xcd)
     return x * per + (x < rem ? x : rem) + i;
 }
 
-template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL>
-__global__ void __launch_bounds__(TPL * NL)
-stockham(fk_args a)
+/*
+ * Body of one workgroup tile: lines [tile*NL, tile*NL + NL).  NT is the
+ * workgroup size; when NT > TPL*NL (persistent workgroups shared by several
+ * geometries) the extra threads work on a scratch LDS line and never touch
+ * global memory, so every thread still meets every barrier.
+ */
+template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT>
+__device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
+                                              unsigned char *smem_raw)
 {
     constexpr int L = RL::L();
+    constexpr int NR = TPL * NL;
     using G = Geo<R, L, TPL, NL, MODE, PAD>;
-    extern __shared__ unsigned char smem_raw[];
+    static_assert(NT >= NR, "workgroup smaller than the geometry");
     cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
     long long *lbase = reinterpret_cast<long long *>(lds + G::lds_elems());
+    /* threads beyond TPL*NL read and write a 64-slot trash area */
+    cx<R> *trash = reinterpret_cast<cx<R> *>(lbase + ((NL + 1) & ~1));
 
     const int tid = threadIdx.x;
-    const int lt = (MODE == FK_ROW) ? tid / TPL : tid % NL;
-    const int u = (MODE == FK_ROW) ? tid % TPL : tid / NL;
-    const long long tile = remap_tile(a.xcd_remap);
+    const bool real = (NT == NR) || tid < NR;
+    const int lt = !real ? 0 : (MODE == FK_ROW) ? tid / TPL : tid % NL;
+    const int u = !real ? (tid - NR) % TPL : (MODE == FK_ROW) ? tid % TPL : tid / NL;
     const long long line = tile * NL + lt;
-    const bool act = line < a.nlines;
+    const bool act = real && line < a.nlines;
 
     long long i0 = 0, i1 = 0, i2 = 0;
     {
@@ -182,7 +194,10 @@ stockham(fk_args a)
     const unsigned fl = a.flags;
     const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
 
-    auto lds_at = [&](int j) -> cx<R> & { return lds[G::idx(lt, j)]; };
+    auto lds_at = [&](int j) -> cx<R> & {
+        if constexpr (NT == NR) return lds[G::idx(lt, j)];
+        else return real ? lds[G::idx(lt, j)] : trash[tid & 63];
+    };
 
     /* output post-op: twiddle, scale, swap */
     auto post = [&](int k, cx<R> v) -> cx<R> {
@@ -205,11 +220,10 @@ stockham(fk_args a)
             run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
         } else {
             auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
-            if (u == 0) lbase[lt] = act ? bout : -1;
+            if (u == 0 && real) lbase[lt] = act ? bout : -1;
             run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
             /* lines-fastest store: consecutive threads write consecutive
                lines of the tile (unit output batch stride) */
-            constexpr int NT = TPL * NL;
             for (int e = tid; e < NL * L; e += NT) {
                 const int l2 = e % NL, k = e / NL;
                 const long long ob = lbase[l2];
@@ -311,10 +325,19 @@ stockham(fk_args a)
 }
 
 template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL>
+__global__ void __launch_bounds__(TPL * NL)
+stockham(fk_args a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    stockham_body<R, NL, MODE, IO, PAD, RL, TPL, TPL * NL>(a, remap_tile(a.xcd_remap), smem_raw);
+}
+
+template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT = TPL * NL>
 constexpr unsigned stockham_lds()
 {
     using G = Geo<R, RL::L(), TPL, NL, MODE, PAD>;
-    return G::lds_elems() * sizeof(cx<R>) + NL * sizeof(long long);
+    return G::lds_elems() * sizeof(cx<R>) + ((NL + 1) & ~1) * sizeof(long long) +
+           (NT > TPL * NL ? 64 * sizeof(cx<R>) : 0);
 }
 
 } // namespace fk

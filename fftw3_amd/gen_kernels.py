@@ -176,7 +176,144 @@ def cxx_entry(e):
     return targs, rl
 
 
+PERSIST_NT = 512
+PERSIST_LDS = 80 * 1024          # two persistent workgroups per CU
+PERSIST_SIZES = {
+    "float": [(n, "ROW", "PLAIN") for n in (64, 128, 256, 512, 1000, 1001, 1024, 2048, 2187, 4096)]
+    + [(n, "COL", "PLAIN") for n in (64, 128, 256, 512, 1024, 2048)]
+    + [(n, "ROW", "TOUT") for n in (64, 128, 256, 512, 1024, 2048, 4096)]
+    + [(n, "ROW", "BLUE") for n in (2048, 4096, 8192)],
+    "double": [(n, "ROW", "PLAIN") for n in (64, 128, 256, 512, 1000, 1001, 1024, 2048, 2187, 4096)]
+    + [(n, "COL", "PLAIN") for n in (64, 128, 256, 512, 1024, 2048)]
+    + [(n, "ROW", "TOUT") for n in (64, 128, 256, 512, 1024, 2048)]
+    + [(n, "ROW", "BLUE") for n in (1024, 2048, 4096)],
+}
+
+
+def persist_menu(prec):
+    """Geometries the persistent kernel carries: default radices and threads
+    per line of the catalog, lines per workgroup grown to fill PERSIST_NT
+    threads within PERSIST_LDS bytes."""
+    out = []
+    for L, mode, io in PERSIST_SIZES[prec]:
+        rads = choose_radices(L)
+        tpl = max(1, L // max(rads))
+        if tpl > PERSIST_NT:
+            continue
+        csz = CSZ[prec]
+        per = (L + L // rads[0] + 2) * csz if mode == "ROW" else (L + 2) * csz
+        nl = max(1, PERSIST_NT // tpl)
+        while nl > 1 and nl * per + 8 * nl + 64 * csz > PERSIST_LDS:
+            nl //= 2
+        if nl * per + 8 * (nl + 1) + 64 * csz > PERSIST_LDS:
+            continue
+        out.append((L, rads, tpl, nl, mode, io))
+    return out
+
+
+def write_persist():
+    """fk_persist_<p>.hip (one per precision) + fk_persist.hip (menu lookup)."""
+    for prec in ("float", "double"):
+        menu = persist_menu(prec)
+        p = PREC[prec]
+        lines = ['// generated by fftw3_amd/gen_kernels.py -- do not edit',
+                 '#include "../fk_stockham.h"', '',
+                 'namespace {', 'constexpr int NT = %d;' % PERSIST_NT, '']
+        # one separately register-allocated function per geometry
+        for i, (L, rads, tpl, nl, mode, io) in enumerate(menu):
+            lines.append('__device__ __noinline__ void body_%d(const fk_args *a, long long tile, unsigned char *smem)' % i)
+            lines.append('{')
+            lines.append('    fk::stockham_body<%s, %d, FK_%s, FK_IO_%s, %d, fk::RadList<%s>, %d, NT>(*a, tile, smem);'
+                         % (prec, nl, mode, io, rads[0], ",".join(map(str, rads)), tpl))
+            lines += ['}', '']
+        lines.append('constexpr unsigned lds_max()')
+        lines.append('{')
+        lines.append('    unsigned m = 0;')
+        for (L, rads, tpl, nl, mode, io) in menu:
+            t = '%s, %d, FK_%s, FK_IO_%s, %d, fk::RadList<%s>, %d, NT' % (
+                prec, nl, mode, io, rads[0], ",".join(map(str, rads)), tpl)
+            lines.append('    m = m > fk::stockham_lds<%s>() ? m : fk::stockham_lds<%s>();' % (t, t))
+        lines += ['    return m;', '}', '']
+        lines.append('const fk_menu_entry menu[] = {')
+        for (L, rads, tpl, nl, mode, io) in menu:
+            lines.append('    {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s},'
+                         % (p, L, len(rads), ",".join(map(str, rads)), tpl, nl, mode, io))
+        lines += ['};', '']
+        lines += [
+            '/* Work-table driven persistent kernel: every workgroup takes the next',
+            '   tile from the global counter until the table is drained (the exit',
+            '   every wave reaches), finds its pass by binary search over the tile',
+            '   prefix sums, and runs that geometry on the tile. */',
+            '__global__ void __launch_bounds__(NT) persist(const fk_bpass *tab, int npass, long long total,',
+            '                                              unsigned long long *counter)',
+            '{',
+            '    extern __shared__ unsigned char smem[];',
+            '    __shared__ long long s_item;',
+            '    for (;;) {',
+            '        if (threadIdx.x == 0) s_item = (long long)atomicAdd(counter, 1ULL);',
+            '        __syncthreads();',
+            '        const long long t = s_item;',
+            '        __syncthreads();',
+            '        if (t >= total) break;',
+            '        int lo = 0, hi = npass - 1;',
+            '        while (lo < hi) {',
+            '            const int mid = (lo + hi + 1) >> 1;',
+            '            if (tab[mid].first <= t) lo = mid; else hi = mid - 1;',
+            '        }',
+            '        const fk_bpass *bp = tab + lo;',
+            '        const long long tile = t - bp->first;',
+            '        switch (bp->kind) {']
+        for i in range(len(menu)):
+            lines.append('        case %d: body_%d(&bp->a, tile, smem); break;' % (i, i))
+        lines += [
+            '        default: break;',
+            '        }',
+            '        __syncthreads();',
+            '    }',
+            '}',
+            '} // namespace',
+            '',
+            'extern "C" const fk_menu_entry *fk_persist_menu_%d(int *n, const void **fn, unsigned *lds)' % p,
+            '{',
+            '    *n = (int)(sizeof(menu) / sizeof(menu[0]));',
+            '    *fn = (const void *)&persist;',
+            '    *lds = lds_max();',
+            '    return menu;',
+            '}',
+            '']
+        path = os.path.join(OUT, "fk_persist_%d.hip" % p)
+        txt = "\n".join(lines)
+        if not os.path.exists(path) or open(path).read() != txt:
+            open(path, "w").write(txt)
+    lines = ['// generated by fftw3_amd/gen_kernels.py -- do not edit',
+             '#include <hip/hip_runtime.h>',
+             '#include "../fk_args.h"', '',
+             'extern "C" const fk_menu_entry *fk_persist_menu_0(int *n, const void **fn, unsigned *lds);',
+             'extern "C" const fk_menu_entry *fk_persist_menu_1(int *n, const void **fn, unsigned *lds);', '',
+             'namespace {',
+             '__global__ void reset_counters(unsigned long long *c, int n)',
+             '{',
+             '    if ((int)threadIdx.x < n) c[threadIdx.x] = 0ULL;',
+             '}',
+             '} // namespace', '',
+             'extern "C" const fk_menu_entry *fk_persist_menu(int prec, int *n, const void **fn, unsigned *lds, int *nt)',
+             '{',
+             '    *nt = %d;' % PERSIST_NT,
+             '    return prec == 0 ? fk_persist_menu_0(n, fn, lds) : fk_persist_menu_1(n, fn, lds);',
+             '}', '',
+             'extern "C" int fkm_reset_counters(unsigned long long *c, int n, hipStream_t st)',
+             '{',
+             '    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, c, n);',
+             '    return (int)hipGetLastError();',
+             '}', '']
+    path = os.path.join(OUT, "fk_persist.hip")
+    txt = "\n".join(lines)
+    if not os.path.exists(path) or open(path).read() != txt:
+        open(path, "w").write(txt)
+
+
 def main():
+    write_persist()
     os.makedirs(OUT, exist_ok=True)
     entries = catalog()
     # variant index within (precision, L, mapping, io): 0 is the default

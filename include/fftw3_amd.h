@@ -60,6 +60,23 @@ double fftw_amd_plan_pass_bytes(const void *plan, int i);
    whole execute.  Negative on error. */
 double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *per_pass_ms);
 
+/* Batched transforms of uneven lengths.  Group g holds howmany[g]
+   contiguous 1-D c2c transforms of length n[g] (distance n[g]) from in[g] to
+   out[g] (device memory).  Every group is planned with `flags` (FFTW_MEASURE
+   tunes each size on the device); the passes of all groups run as one
+   persistent, work-table driven launch per phase, recorded once into a
+   hipGraph and replayed by fftw_amd_batch_execute.  prec: 0 double, 1 float.
+   Returns NULL on invalid arguments or allocation failure. */
+typedef struct fftw_amd_batch_s *fftw_amd_batch;
+fftw_amd_batch fftw_amd_plan_batch_dft_1d(int prec, int ngroups, const int *n, const int *howmany,
+                                          void *const *in, void *const *out, int sign,
+                                          unsigned flags);
+int fftw_amd_batch_execute(fftw_amd_batch b);         /* graph replay */
+int fftw_amd_batch_execute_direct(fftw_amd_batch b);  /* same launches, no graph */
+int fftw_amd_batch_info(fftw_amd_batch b, char *buf, int len);
+void *fftw_amd_batch_stream(fftw_amd_batch b);
+void fftw_amd_batch_destroy(fftw_amd_batch b);
+
 /* Events on the current stream for timing regions (milliseconds). */
 void *fftw_amd_event_create(void);
 void fftw_amd_event_record(void *ev, void *hip_stream);
