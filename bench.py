@@ -50,7 +50,251 @@ def parse():
                     help="plan with FFTW_ESTIMATE instead of FFTW_MEASURE (device autotuning)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 2 GiB rank-0 copy plan (known traffic) for counter calibration")
+    ap.add_argument("--workload", default="multid", choices=["multid", "mixed", "serve"],
+                    help="multid: the default bench line; mixed: uneven-size batching study; "
+                         "serve: serving loop latency/throughput study")
+    ap.add_argument("--requests", type=int, default=1500)
+    ap.add_argument("--load", type=float, default=0.7, help="offered load as a fraction of capacity")
     return ap.parse_args()
+
+
+def serving_requests(count, seed=5):
+    """Request stream: 70% 1-D c2c, 20% 2-D r2c, 10% 3-D c2c (64^3..256^3);
+    N log-uniform in 2^8..2^22 with ~10% non-powers of two or primes; 80%
+    FP32 / 20% FP64; howmany log-uniform in 1..256, capped at 2^24 points."""
+    import oracle
+    u = (oracle.uniform(8 * count, seed, 11) + 1.0) / 2.0
+    odd = [3, 5, 7, 9, 15]
+    primes = [257, 1009, 4099, 16411, 65537, 262147]
+    reqs = []
+    for i in range(count):
+        a, b, c, d, e, f, g, h = u[8 * i: 8 * i + 8]
+        prec = "float" if b < 0.8 else "double"
+        if a < 0.7:
+            k = 8 + int(c * 15)
+            n = 1 << k
+            if d < 0.1:
+                if e < 0.5:
+                    m = odd[int(f * len(odd))]
+                    n = m << max(0, k - 3)
+                else:
+                    n = primes[int(f * len(primes))]
+            shape, kind = (n,), "c2c"
+        elif a < 0.9:
+            k = 8 + int(c * 15)
+            shape, kind = (1 << (k // 2), 1 << (k - k // 2)), "r2c"
+        else:
+            s = [64, 128, 256][int(c * 3)]
+            shape, kind = (s, s, s), "c2c"
+        N = int(np.prod(shape))
+        hm = int(math.exp(g * math.log(256.0)))
+        hm = max(1, min(hm, (1 << 24) // N))
+        reqs.append((kind, prec, shape, hm))
+    return reqs
+
+
+def run_serve(args):
+    """Serving loop: capacity with everything queued at once, then Poisson
+    arrivals at `--load` x capacity; reports p50/p99 latency and FFTs/s."""
+    import threading
+    import fftw3_amd as fa
+    import oracle
+    fa.init()
+    reqs = serving_requests(args.requests)
+    shapes = {}
+    for kind, prec, shape, hm in reqs:
+        key = (kind, prec, shape)
+        shapes[key] = max(shapes.get(key, 0), hm)
+    inbuf = {}
+    for i, ((kind, prec, shape), hm) in enumerate(sorted(shapes.items(), key=str)):
+        N = int(np.prod(shape)) * hm
+        reals = 2 * N if kind == "c2c" else N
+        b = fa.DeviceBuffer(reals * (8 if prec == "double" else 4))
+        b.fill_uniform(prec, reals, 5, i)
+        inbuf[(kind, prec, shape)] = b
+    budget = 8 << 30
+    srv = fa.Server(budget, max_rounds=4, per_round=32)
+    t0 = time.time()
+    reqobjs = []
+    for kind, prec, shape, hm in reqs:
+        r = fa.Server.request(kind, prec, shape, hm, inbuf[(kind, prec, shape)].ptr)
+        reqobjs.append(r)
+    for r in reqobjs:
+        srv.prepare(r)
+    log("serving: %d requests, %d shapes, planned in %.2fs" % (len(reqs), len(shapes), time.time() - t0))
+    transforms = sum(hm for _, _, _, hm in reqs)
+
+    def drain(ids):
+        for i in ids:
+            srv.wait(i)
+            srv.release(i)
+
+    # capacity: everything submitted at once
+    t = time.time()
+    ids = [srv.submit(r) for r in reqobjs]
+    drain(ids)
+    cap_s = time.time() - t
+    cap_rps = len(reqs) / cap_s
+    log("capacity: %.3f s for %d requests (%.1f req/s, %.1f FFTs/s)" %
+        (cap_s, len(reqs), cap_rps, transforms / cap_s))
+    srv.destroy()
+    # Poisson arrivals at the offered rate
+    lam = args.load * cap_rps
+    srv = fa.Server(budget, max_rounds=4, per_round=32)
+    for r in reqobjs:
+        srv.prepare(r)
+    gaps = -np.log(1.0 - (oracle.uniform(len(reqs), 5, 13) + 1.0) / 2.0) / lam
+    arrivals = np.cumsum(gaps)
+    ids = []
+    done = threading.Event()
+
+    def consumer():
+        k = 0
+        while k < len(reqs):
+            if k < len(ids):
+                srv.wait(ids[k])
+                srv.release(ids[k])
+                k += 1
+            else:
+                time.sleep(0.0002)
+        done.set()
+
+    th = threading.Thread(target=consumer)
+    th.start()
+    start = time.time()
+    for r, ta in zip(reqobjs, arrivals):
+        delay = start + ta - time.time()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(r))
+    th.join()
+    wall = time.time() - start
+    st = srv.stats()
+    srv.destroy()
+    rec = {"metric": "served FFTs/s at p99 latency (mixed serving stream, 1 GPU)",
+           "value": round(st["transforms"] / st["elapsed_s"], 1), "unit": "FFTs/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f32 (80%) / f64 (20%)",
+           "data": "synthetic seed 5",
+           "config": {"workload": "serving mix: 70% 1-D c2c, 20% 2-D r2c, 10% 3-D c2c; N 2^8..2^22",
+                      "requests": len(reqs), "transforms": transforms, "offered_rate_req_s": round(lam, 1),
+                      "load_fraction": args.load, "hbm_budget_bytes": budget, "rounds_in_flight": 4,
+                      "requests_per_round": 32},
+           "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+           "requests_per_s": round(st["completed"] / st["elapsed_s"], 1),
+           "capacity_req_s": round(cap_rps, 1), "wall_s": round(wall, 3)}
+    log("p50 %.3f ms  p99 %.3f ms  %.1f FFTs/s at %.1f req/s offered" %
+        (st["p50_ms"], st["p99_ms"], rec["value"], lam))
+    print(json.dumps(rec), flush=True)
+
+
+def mixed_sizes(total_log2=28, seed=3):
+    """Uneven-size set: N in {1000, 1001, 2187, 4093, 4096, 65536, 2^20},
+    batch b_N = floor(exp(U(0, ln 4096))) from the counter-based stream,
+    rescaled so that sum N*b_N ~ 2^total_log2 with b_N >= 1."""
+    import oracle
+    sizes = [1000, 1001, 2187, 4093, 4096, 65536, 1 << 20]
+    u = (oracle.uniform(len(sizes), seed, 7) + 1.0) / 2.0
+    b = [int(math.exp(x * math.log(4096.0))) for x in u]
+    tot = sum(n * k for n, k in zip(sizes, b))
+    s = float(1 << total_log2) / tot
+    b = [max(1, int(round(k * s))) for k in b]
+    return list(zip(sizes, b))
+
+
+def run_mixed(args):
+    """Throughput of the uneven-size set three ways: one plan call per
+    transform, one batched plan per size, and the persistent work-table
+    batch replayed through a hipGraph (FP32 c2c forward)."""
+    import fftw3_amd as fa
+    import oracle
+    fa.init()
+    L = fa.lib()
+    L.fftw_amd_set_async(1)
+    mix = mixed_sizes()
+    ntr = sum(k for _, k in mix)
+    pts = sum(n * k for n, k in mix)
+    log("mixed set (N, howmany): %s; %d transforms, %d points" % (mix, ntr, pts))
+    bufs = []
+    for i, (n, k) in enumerate(mix):
+        bi = fa.DeviceBuffer(n * k * 8)
+        bi.fill_uniform("float", 2 * n * k, 3, i)
+        bo = fa.DeviceBuffer(n * k * 8)
+        bufs.append((bi, bo))
+    e0, e1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
+
+    def timed(fn, reps):
+        fn()
+        L.fftw_amd_synchronize()
+        best = 1e30
+        for _ in range(reps):
+            L.fftw_amd_event_record(e0, None)
+            fn()
+            L.fftw_amd_event_record(e1, None)
+            L.fftw_amd_synchronize()
+            best = min(best, L.fftw_amd_event_elapsed(e0, e1))
+        return best
+
+    # (1) one plan per size with howmany=1, one execute per transform
+    single = []
+    for (n, k), (bi, bo) in zip(mix, bufs):
+        single.append(fa.plan_many_dft("float", [n], 1, bi.ptr, None, 1, n, bo.ptr, None, 1, n,
+                                       fa.FFTW_FORWARD, fa.FFTW_ESTIMATE))
+
+    def per_call():
+        for (n, k), (bi, bo), p in zip(mix, bufs, single):
+            for t in range(k):
+                p.execute_dft(bi.ptr + t * n * 8, bo.ptr + t * n * 8)
+
+    t1 = timed(per_call, 2)
+    # (2) one batched plan per size
+    many = [fa.plan_many_dft("float", [n], k, bi.ptr, None, 1, n, bo.ptr, None, 1, n,
+                             fa.FFTW_FORWARD, fa.FFTW_ESTIMATE) for (n, k), (bi, bo) in zip(mix, bufs)]
+
+    def per_size():
+        for p in many:
+            p.execute()
+
+    t2 = timed(per_size, 3)
+    # (3) persistent work-table launches in a hipGraph
+    groups = [(n, k, bi.ptr, bo.ptr) for (n, k), (bi, bo) in zip(mix, bufs)]
+    bt = fa.Batch("float", groups, fa.FFTW_FORWARD, fa.FFTW_ESTIMATE)
+    log(bt.info())
+    stream = bt.stream()
+
+    def batched():
+        bt.execute()
+
+    # the batch runs on its own stream: time it with events on that stream
+    bt.execute()
+    L.fftw_amd_synchronize()
+    t3 = 1e30
+    for _ in range(5):
+        L.fftw_amd_event_record(e0, stream)
+        bt.execute()
+        L.fftw_amd_event_record(e1, stream)
+        L.fftw_amd_synchronize()
+        t3 = min(t3, L.fftw_amd_event_elapsed(e0, e1))
+    # accuracy of the batched result: first transform of every size
+    errs = {}
+    for (n, k), (bi, bo) in zip(mix, bufs):
+        x = bi.download(np.complex64, n).astype(np.complex128)
+        y = bo.download(np.complex64, n)
+        errs[n] = oracle.rel_l2(y, oracle.fftn(x, -1))
+    byts = 2.0 * pts * 8
+    res = {}
+    for name, t in (("per_call", t1), ("per_size_batched", t2), ("persistent_graph", t3)):
+        res[name] = {"ms": round(t, 3), "transforms_per_s": round(ntr / (t * 1e-3), 1),
+                     "GB_s": round(byts / (t * 1e-3) / 1e9, 1)}
+        log("%-18s %9.3f ms  %12.1f transforms/s  %8.1f GB/s" % (name, t, ntr / (t * 1e-3),
+                                                                  byts / (t * 1e-3) / 1e9))
+    rec = {"metric": "transforms/s (uneven-size FP32 c2c set, 1 GPU)",
+           "value": res["persistent_graph"]["transforms_per_s"], "unit": "transforms/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic seed 3",
+           "config": {"workload": "mixed N in {1000,1001,2187,4093,4096,65536,2^20}",
+                      "sizes_howmany": mix, "points": pts},
+           "speedup_vs_per_call": round(t1 / t3, 2), "speedup_vs_per_size": round(t2 / t3, 2),
+           "modes": res, "rel_l2_first_transform": errs}
+    print(json.dumps(rec), flush=True)
 
 
 def main():
@@ -59,6 +303,12 @@ def main():
         print("bench.py measures one GPU only; this build has no multi-GPU path "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mixed":
+        run_mixed(args)
+        return
+    if args.workload == "serve":
+        run_serve(args)
+        return
     import fftw3_amd as fa
     import oracle  # checker only: round-trip references and cpu_baseline
 

@@ -35,6 +35,75 @@ class IODim(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("is_", ctypes.c_int), ("os", ctypes.c_int)]
 
 
+class Request(ctypes.Structure):
+    """fftw_amd_request: kind 0 c2c / 1 r2c / 2 c2r, prec 0 double / 1 float."""
+    _fields_ = [("kind", ctypes.c_int), ("prec", ctypes.c_int), ("sign", ctypes.c_int),
+                ("rank", ctypes.c_int), ("n", ctypes.c_int * 3), ("howmany", ctypes.c_int),
+                ("in_", ctypes.c_void_p)]
+
+
+class Server:
+    """fftw_amd_server handle (continuous-batching serving loop)."""
+
+    def __init__(self, budget_bytes, max_rounds=4, per_round=64):
+        self.h = lib().fftw_amd_server_create(int(budget_bytes), max_rounds, per_round)
+        if not self.h:
+            raise RuntimeError("server creation failed: " + last_error())
+
+    @staticmethod
+    def request(kind, prec, shape, howmany, inp, sign=-1):
+        r = Request()
+        r.kind = {"c2c": 0, "r2c": 1, "c2r": 2}[kind]
+        r.prec = 0 if prec == "double" else 1
+        r.sign = sign
+        r.rank = len(shape)
+        for i, v in enumerate(shape):
+            r.n[i] = v
+        r.howmany = howmany
+        r.in_ = inp
+        return r
+
+    def prepare(self, r):
+        return lib().fftw_amd_server_prepare(self.h, ctypes.byref(r))
+
+    def submit(self, r):
+        return lib().fftw_amd_server_submit(self.h, ctypes.byref(r))
+
+    def poll(self, i):
+        return lib().fftw_amd_server_poll(self.h, i)
+
+    def wait(self, i):
+        out = ctypes.c_void_p()
+        lat = ctypes.c_double()
+        rc = lib().fftw_amd_server_wait(self.h, i, ctypes.byref(out), ctypes.byref(lat))
+        if rc:
+            raise RuntimeError("request %d failed" % i)
+        return out.value, lat.value
+
+    def release(self, i):
+        return lib().fftw_amd_server_release(self.h, i)
+
+    def stats(self):
+        p50, p99 = ctypes.c_double(), ctypes.c_double()
+        done, tr = ctypes.c_longlong(), ctypes.c_longlong()
+        el, used = ctypes.c_double(), ctypes.c_size_t()
+        lib().fftw_amd_server_stats(self.h, ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(done),
+                                    ctypes.byref(tr), ctypes.byref(el), ctypes.byref(used))
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "completed": done.value,
+                "transforms": tr.value, "elapsed_s": el.value, "hbm_in_use": used.value}
+
+    def destroy(self):
+        if self.h:
+            lib().fftw_amd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class IODim64(ctypes.Structure):
     _fields_ = [("n", ctypes.c_ssize_t), ("is_", ctypes.c_ssize_t), ("os", ctypes.c_ssize_t)]
 
@@ -141,6 +210,19 @@ def lib():
     L.fftw_amd_batch_stream.argtypes = [vp]
     L.fftw_amd_batch_stream.restype = vp
     L.fftw_amd_batch_destroy.argtypes = [vp]
+    L.fftw_amd_server_create.restype = vp
+    L.fftw_amd_server_create.argtypes = [ctypes.c_size_t, ip, ip]
+    L.fftw_amd_server_prepare.argtypes = [vp, ctypes.POINTER(Request)]
+    L.fftw_amd_server_submit.argtypes = [vp, ctypes.POINTER(Request)]
+    L.fftw_amd_server_submit.restype = ctypes.c_longlong
+    L.fftw_amd_server_poll.argtypes = [vp, ctypes.c_longlong]
+    L.fftw_amd_server_wait.argtypes = [vp, ctypes.c_longlong, ctypes.POINTER(vp),
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.fftw_amd_server_release.argtypes = [vp, ctypes.c_longlong]
+    L.fftw_amd_server_stats.argtypes = [vp] + [ctypes.POINTER(ctypes.c_double)] * 2 + \
+        [ctypes.POINTER(ctypes.c_longlong)] * 2 + [ctypes.POINTER(ctypes.c_double),
+                                                   ctypes.POINTER(ctypes.c_size_t)]
+    L.fftw_amd_server_destroy.argtypes = [vp]
     L.fftw_amd_event_create.restype = vp
     L.fftw_amd_event_record.argtypes = [vp, vp]
     L.fftw_amd_event_elapsed.argtypes = [vp, vp]

@@ -699,6 +699,34 @@ int fftw_amd_batch_info(fftw_amd_batch b, char *buf, int len)
 void *fftw_amd_batch_stream(fftw_amd_batch b) { return (void *)fk_batch_stream((fk_batch *)b); }
 void fftw_amd_batch_destroy(fftw_amd_batch b) { fk_batch_destroy((fk_batch *)b); }
 
+fftw_amd_server fftw_amd_server_create(size_t budget, int max_rounds, int per_round)
+{
+    return (fftw_amd_server)fk_server_create(budget, max_rounds, per_round);
+}
+int fftw_amd_server_prepare(fftw_amd_server s, const fftw_amd_request *r)
+{
+    return fk_server_prepare((fk_server *)s, (const fk_request *)r);
+}
+long long fftw_amd_server_submit(fftw_amd_server s, const fftw_amd_request *r)
+{
+    return fk_server_submit((fk_server *)s, (const fk_request *)r);
+}
+int fftw_amd_server_poll(fftw_amd_server s, long long id) { return fk_server_poll((fk_server *)s, id); }
+int fftw_amd_server_wait(fftw_amd_server s, long long id, void **out, double *lat)
+{
+    return fk_server_wait((fk_server *)s, id, out, lat);
+}
+int fftw_amd_server_release(fftw_amd_server s, long long id)
+{
+    return fk_server_release((fk_server *)s, id);
+}
+int fftw_amd_server_stats(fftw_amd_server s, double *p50, double *p99, long long *done,
+                          long long *transforms, double *elapsed, size_t *inuse)
+{
+    return fk_server_stats((fk_server *)s, p50, p99, done, transforms, elapsed, inuse);
+}
+void fftw_amd_server_destroy(fftw_amd_server s) { fk_server_destroy((fk_server *)s); }
+
 void *fftw_amd_event_create(void)
 {
     hipEvent_t e = 0;

@@ -41,7 +41,8 @@ typedef struct fk_batch {
     const void *fn;
     unsigned lds;
     int nt, grid;
-    hipStream_t st;
+    hipStream_t st, st2;
+    hipEvent_t ev_fork, ev_join;
     hipGraph_t graph;
     hipGraphExec_t exec;
     long long ntransforms;
@@ -57,7 +58,7 @@ static int same_rad(const fk_menu_entry *m, const fk_kernel *k)
     return 1;
 }
 
-static int menu_kind(const fk_menu_entry *menu, int nm, const fk_kernel *k)
+int fk_menu_kind(const fk_menu_entry *menu, int nm, const fk_kernel *k)
 {
     int i;
     for (i = 0; i < nm; ++i)
@@ -69,7 +70,17 @@ static int menu_kind(const fk_menu_entry *menu, int nm, const fk_kernel *k)
 
 static int enqueue_all(fk_batch *b)
 {
-    int p, g, rc = 0;
+    int p, g, rc = 0, nside = 0;
+    for (g = 0; g < b->ngroups; ++g) nside += !b->persist[g];
+    /* fork: groups with their own launches run on the side stream,
+       concurrently with the persistent phases */
+    if (nside) {
+        rc = (int)hipEventRecord(b->ev_fork, b->st);
+        if (!rc) rc = (int)hipStreamWaitEvent(b->st2, b->ev_fork, 0);
+        for (g = 0; g < b->ngroups && !rc; ++g)
+            if (!b->persist[g]) rc = fk_enqueue(b->plans[g], b->in[g], b->out[g], b->st2);
+        if (!rc) rc = (int)hipEventRecord(b->ev_join, b->st2);
+    }
     for (p = 0; p < b->nphase && !rc; ++p) {
         void *args[4];
         int np = b->npass[p];
@@ -86,8 +97,8 @@ static int enqueue_all(fk_batch *b)
                                           (dim3){(unsigned)b->nt, 1, 1}, args, b->lds, b->st);
         }
     }
-    for (g = 0; g < b->ngroups && !rc; ++g)
-        if (!b->persist[g]) rc = fk_enqueue(b->plans[g], b->in[g], b->out[g], b->st);
+    /* join */
+    if (nside && !rc) rc = (int)hipStreamWaitEvent(b->st, b->ev_join, 0);
     return rc;
 }
 
@@ -107,6 +118,9 @@ void fk_batch_destroy(fk_batch *b)
     free(b->persist);
     free(b->in);
     free(b->out);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->st2) (void)hipStreamDestroy(b->st2);
     if (b->st) (void)hipStreamDestroy(b->st);
     free(b);
 }
@@ -127,6 +141,12 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
     b->out = (void **)calloc(ngroups, sizeof(void *));
     menu = fk_persist_menu(prec, &nm, &b->fn, &b->lds, &b->nt);
     if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) goto fail;
+    if (hipStreamCreateWithFlags(&b->st2, hipStreamNonBlocking) != hipSuccess) goto fail;
+    if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) goto fail;
+    if (hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) goto fail;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+        ncu <= 0)
+        ncu = 256;
     if (b->lds > 65536 &&
         hipFuncSetAttribute(b->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds) !=
             hipSuccess)
@@ -154,11 +174,20 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
         b->out[g] = out[g];
         b->ntransforms += howmany[g];
         b->bytes += 2.0 * (double)n[g] * howmany[g] * (prec == 0 ? 16 : 8);
-        /* all passes in the menu and at most MAXPH of them? */
+        /* all passes in the menu, at most MAXPH of them, and the group too
+           small to fill the GPU with its own tuned launches?  Groups that
+           fill it alone (every pass >= 4 tiles per CU) keep their own
+           kernels on a parallel branch. */
         b->persist[g] = b->plans[g]->npass > 0 && b->plans[g]->npass <= MAXPH;
-        for (i = 0; i < b->plans[g]->npass && b->persist[g]; ++i) {
-            const fk_pass *ps = b->plans[g]->pass + i;
-            if (ps->type != FK_PASS_STOCKHAM || menu_kind(menu, nm, ps->k) < 0) b->persist[g] = 0;
+        {
+            int big = 1;
+            for (i = 0; i < b->plans[g]->npass; ++i) {
+                const fk_pass *ps = b->plans[g]->pass + i;
+                if (ps->type != FK_PASS_STOCKHAM || fk_menu_kind(menu, nm, ps->k) < 0)
+                    b->persist[g] = 0;
+                if (ps->type != FK_PASS_STOCKHAM || ps->grid < 4u * (unsigned)ncu) big = 0;
+            }
+            if (big) b->persist[g] = 0;
         }
     }
     /* work tables, one per phase */
@@ -171,7 +200,7 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
         for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = P->scratch[i];
         for (i = 0; i < P->npass; ++i) {
             const fk_pass *ps = P->pass + i;
-            int kind = menu_kind(menu, nm, ps->k);
+            int kind = fk_menu_kind(menu, nm, ps->k);
             fk_bpass *e;
             p = i;
             b->htab[p] = (fk_bpass *)realloc(b->htab[p], sizeof(fk_bpass) * (b->npass[p] + 1));
@@ -196,10 +225,15 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
     }
     b->dcount = (unsigned long long *)fk_malloc(sizeof(unsigned long long) * MAXPH);
     if (!b->dcount) goto fail;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
-        ncu <= 0)
-        ncu = 256;
-    b->grid = ncu * (b->lds <= 81920 ? 2 : 1);
+    {
+        /* one resident wave of persistent workgroups */
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn, b->nt, b->lds) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        b->grid = ncu * per_cu;
+    }
     /* record the launch sequence once */
     if (hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal) != hipSuccess) goto fail;
     if (enqueue_all(b)) {
@@ -234,8 +268,9 @@ int fk_batch_info(fk_batch *b, char *buf, int len)
     if (!b) return -1;
     for (g = 0; g < b->ngroups; ++g) np += b->persist[g];
     off += snprintf(buf + off, len - off,
-                    "batch: %d groups (%d in persistent phases), %lld transforms, %d phases",
-                    b->ngroups, np, b->ntransforms, b->nphase);
+                    "batch: %d groups (%d in persistent phases, %d on the parallel branch), "
+                    "%lld transforms, %d phases",
+                    b->ngroups, np, b->ngroups - np, b->ntransforms, b->nphase);
     for (g = 0; g < b->nphase && off < len; ++g)
         off += snprintf(buf + off, len - off, "; phase %d: %d passes %lld tiles", g, b->npass[g],
                         b->total[g]);

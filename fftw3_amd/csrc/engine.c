@@ -1416,11 +1416,18 @@ int fk_execute(fk_plan *p, void *in, void *out)
    synchronisation (usable inside stream capture). */
 int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st)
 {
+    return fk_enqueue_bound(p, in, out, p->scratch, st);
+}
+
+/* Same with caller-owned scratch slots (scratch[i] >= p->scratch_bytes[i]):
+   several executions of one plan may then overlap on the device. */
+int fk_enqueue_bound(fk_plan *p, void *in, void *out, void *const *scratch, hipStream_t st)
+{
     void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
     int i, rc = 0;
     buf[FK_BUF_IN] = in;
     buf[FK_BUF_OUT] = out;
-    for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = scratch[i];
     for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
     return rc;
 }

@@ -131,7 +131,26 @@ int fk_batch_info(fk_batch *b, char *buf, int len);
 hipStream_t fk_batch_stream(fk_batch *b);
 void fk_batch_destroy(fk_batch *b);
 int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st);
+int fk_enqueue_bound(fk_plan *p, void *in, void *out, void *const *scratch, hipStream_t st);
 int fkm_reset_counters(unsigned long long *c, int n, hipStream_t st);
+
+/* serving loop (serve.c) */
+typedef struct {
+    int kind, prec, sign, rank;
+    int n[3];
+    int howmany;
+    const void *in;
+} fk_request;
+typedef struct fk_server fk_server;
+fk_server *fk_server_create(size_t budget, int max_rounds, int per_round);
+int fk_server_prepare(fk_server *s, const fk_request *r);
+long long fk_server_submit(fk_server *s, const fk_request *r);
+int fk_server_poll(fk_server *s, long long id);
+int fk_server_wait(fk_server *s, long long id, void **out, double *lat_ms);
+int fk_server_release(fk_server *s, long long id);
+int fk_server_stats(fk_server *s, double *p50, double *p99, long long *done, long long *transforms,
+                    double *elapsed_s, size_t *hbm_in_use);
+void fk_server_destroy(fk_server *s);
 
 /* launchers from fk_misc.hip */
 int fkm_gen_stage_tw(void *out, int prec, int nrad, const int *rad, int total, hipStream_t st);

@@ -77,6 +77,38 @@ int fftw_amd_batch_info(fftw_amd_batch b, char *buf, int len);
 void *fftw_amd_batch_stream(fftw_amd_batch b);
 void fftw_amd_batch_destroy(fftw_amd_batch b);
 
+/* Serving loop with continuous batching.
+   A request: kind (0 c2c, 1 r2c, 2 c2r), prec (0 double, 1 float), sign
+   (c2c), rank 1..3 with sizes n[], howmany contiguous row-major transforms
+   (complex side of r2c/c2r has n/2+1 along the last dim), input in device
+   memory (unchanged until the request completes; multi-dimensional c2r
+   destroys it, as in FFTW).  The server owns the outputs: an arena of
+   `hbm_budget` bytes holds every admitted request's output and scratch;
+   requests are admitted in arrival order as budget and round slots free
+   up; up to `max_rounds` rounds are in flight, each with up to
+   `per_round` requests, whose menu-compatible passes run as one persistent
+   launch per phase and precision. */
+typedef struct {
+    int kind, prec, sign, rank;
+    int n[3];
+    int howmany;
+    const void *in;
+} fftw_amd_request;
+typedef struct fftw_amd_server_s *fftw_amd_server;
+fftw_amd_server fftw_amd_server_create(size_t hbm_budget, int max_rounds, int per_round);
+/* plan a request shape ahead of time (plans stay resident); 0 on success */
+int fftw_amd_server_prepare(fftw_amd_server s, const fftw_amd_request *r);
+/* request id >= 0; -1 if invalid or larger than the whole budget */
+long long fftw_amd_server_submit(fftw_amd_server s, const fftw_amd_request *r);
+int fftw_amd_server_poll(fftw_amd_server s, long long id);    /* 1 done, 0 not yet, -1 error */
+/* block until done; *out = device pointer of the result, *latency_ms =
+   completion - submission */
+int fftw_amd_server_wait(fftw_amd_server s, long long id, void **out, double *latency_ms);
+int fftw_amd_server_release(fftw_amd_server s, long long id); /* returns the output's bytes */
+int fftw_amd_server_stats(fftw_amd_server s, double *p50_ms, double *p99_ms, long long *completed,
+                          long long *transforms, double *elapsed_s, size_t *hbm_in_use);
+void fftw_amd_server_destroy(fftw_amd_server s);
+
 /* Events on the current stream for timing regions (milliseconds). */
 void *fftw_amd_event_create(void);
 void fftw_amd_event_record(void *ev, void *hip_stream);
