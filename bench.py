@@ -50,7 +50,7 @@ def parse():
                     help="plan with FFTW_ESTIMATE instead of FFTW_MEASURE (device autotuning)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 2 GiB rank-0 copy plan (known traffic) for counter calibration")
-    ap.add_argument("--workload", default="multid", choices=["multid", "mixed", "serve"],
+    ap.add_argument("--workload", default="multid", choices=["multid", "mixed", "serve", "precision"],
                     help="multid: the default bench line; mixed: uneven-size batching study; "
                          "serve: serving loop latency/throughput study")
     ap.add_argument("--requests", type=int, default=1500)
@@ -91,6 +91,53 @@ def serving_requests(count, seed=5):
         hm = max(1, min(hm, (1 << 24) // N))
         reqs.append((kind, prec, shape, hm))
     return reqs
+
+
+def run_precision(args):
+    """Precision study: c2c N=2^24, howmany=8, Gaussian N(0,1) (seed 4):
+    FP64, FP32 (FP64-generated twiddles) and FP32 with FP16 hi/lo
+    matrix-core radix-16 passes, each against the long-double oracle."""
+    import fftw3_amd as fa
+    import oracle
+    fa.init()
+    L = fa.lib()
+    n, H = 1 << 24, 8
+    t = time.time()
+    x64 = oracle.gauss(2 * n * H, 4).view(np.complex128)
+    ref = oracle.fftn(x64.reshape(H, n), -1, axes_batch=1).reshape(-1)
+    log("oracle (long double) %.1fs" % (time.time() - t))
+    refmax = float(np.max(np.abs(ref)))
+    e0, e1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
+    out = {}
+    for name, prec, mf in (("fp64", "double", False), ("fp32", "float", False),
+                           ("fp32_fp16x2_mfma", "float", True)):
+        x = x64 if prec == "double" else x64.astype(np.complex64)
+        a = fa.DeviceBuffer.from_array(x)
+        b = fa.DeviceBuffer(x.nbytes)
+        fl = fa.FFTW_ESTIMATE | (fa.FFTW_AMD_MFMA if mf else 0)
+        p = fa.plan_many_dft(prec, [n], H, a.ptr, None, 1, n, b.ptr, None, 1, n, -1, fl)
+        sw = sum(1 for s_ in p.pass_info() if "tpl=512 nl=1" in s_ or "L=256" in s_) if mf else 0
+        p.execute()
+        y = b.download(x.dtype)
+        err = oracle.rel_l2(y, ref)
+        emax = float(np.max(np.abs(y.astype(np.complex128) - ref))) / refmax
+        tot, per = p.time(a.ptr, b.ptr, iters=5)
+        byts = 2.0 * n * H * np.dtype(x.dtype).itemsize
+        rec = {"rel_l2": err, "max_err_rel": emax, "ms": round(tot, 4),
+               "GB_s": round(byts / (tot * 1e-3) / 1e9, 1), "passes": p.pass_info()}
+        if mf:
+            # per 4096-point line: 3 stages x 256 butterflies x 3 split
+            # products x (32 x 32 x 2) flops; two passes of n*H/4096 lines
+            flops = 2.0 * (n * H / 4096) * 3 * 256 * 3 * 32 * 32 * 2
+            rec["mfma_passes"] = sw
+            rec["mfma_dense_tflops"] = round(flops / (tot * 1e-3) / 1e12, 2)
+            rec["mfma_utilization_of_2.5PF"] = round(flops / (tot * 1e-3) / 2.5e15, 4)
+        out[name] = rec
+        log("%-18s rel_l2 %.3e  max %.3e  %.3f ms  %.1f GB/s" % (name, err, emax, tot, rec["GB_s"]))
+        del a, b, p
+    print(json.dumps({"metric": "precision study rel-L2 vs long-double oracle",
+                      "config": {"workload": "c2c N=2^24 howmany=8 Gaussian seed 4"},
+                      "variants": out}), flush=True)
 
 
 def run_serve(args):
@@ -308,6 +355,9 @@ def main():
         return
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "precision":
+        run_precision(args)
         return
     import fftw3_amd as fa
     import oracle  # checker only: round-trip references and cpu_baseline

@@ -27,6 +27,7 @@ FFTW_PRESERVE_INPUT = 1 << 4
 FFTW_PATIENT = 1 << 5
 FFTW_ESTIMATE = 1 << 6
 FFTW_WISDOM_ONLY = 1 << 21
+FFTW_AMD_MFMA = 1 << 24
 
 _lib = None
 
@@ -210,6 +211,8 @@ def lib():
     L.fftw_amd_batch_stream.argtypes = [vp]
     L.fftw_amd_batch_stream.restype = vp
     L.fftw_amd_batch_destroy.argtypes = [vp]
+    L.fftw_amd_plan_use_mfma.argtypes = [vp]
+    L.fftw_amd_mfma_probe.argtypes = [vp]
     L.fftw_amd_server_create.restype = vp
     L.fftw_amd_server_create.argtypes = [ctypes.c_size_t, ip, ip]
     L.fftw_amd_server_prepare.argtypes = [vp, ctypes.POINTER(Request)]

@@ -684,6 +684,13 @@ double fftw_amd_plan_time(void *p, void *in, void *out, int iters, double *per_p
     return p ? fk_plan_time((fk_plan *)p, in, out, iters, per_pass_ms) : -1;
 }
 
+int fftw_amd_plan_use_mfma(void *p) { return fk_plan_use_mfma((fk_plan *)p); }
+int fftw_amd_mfma_probe(float *dev_out)
+{
+    int rc = fkm_mfma_probe(dev_out, 0);
+    return rc ? rc : (int)hipDeviceSynchronize();
+}
+
 fftw_amd_batch fftw_amd_plan_batch_dft_1d(int prec, int ngroups, const int *n,
                                           const int *howmany, void *const *in, void *const *out,
                                           int sign, unsigned flags)

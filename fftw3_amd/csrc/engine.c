@@ -264,6 +264,12 @@ static const fk_kernel *find_kernel(int prec, long long L, int mode, int io)
 static int has_single(int prec, long long n) { return find_kernel(prec, n, FK_ROW, FK_IO_PLAIN) != 0; }
 
 /* n decomposable into catalog passes without Bluestein */
+static int is_pow16(long long n)
+{
+    while (n > 1 && n % 16 == 0) n /= 16;
+    return n == 1;
+}
+
 static int smooth_plannable(int prec, long long n, int depth)
 {
     long long n1;
@@ -293,6 +299,7 @@ typedef struct {
     const int *choice;          /* decisions to replay (NULL: all defaults) */
     int nchoice;
     int ci;                     /* decisions taken so far */
+    int r16;                    /* restrict to radix-16 passes (FK_MFMA) */
     int nalt[FK_MAX_DECIDE];    /* alternatives at each decision */
     int dkind[FK_MAX_DECIDE];
 } ctx;
@@ -620,9 +627,18 @@ static int plan_bluestein(ctx *c, long long n, long long is, long long os, int n
 }
 
 /* pick n1 (column pass length) for a two-pass split of n */
-/* Candidate column lengths n1 for a two-pass split of n, best first:
-   balanced factors, a single-pass second factor, and column tiles of at
-   least 16 lines preferred.  Returns the number of candidates (<= max). */
+/* Relative cost of an HBM pass whose contiguous runs are `seg` bytes
+   (one 128-byte line and up stream well; shorter runs waste bursts). */
+static double seg_pen(double seg)
+{
+    return seg >= 256 ? 0.0 : seg >= 128 ? 0.15 : seg >= 64 ? 0.5 : seg >= 32 ? 1.5 : 3.0;
+}
+
+/* Candidate column lengths n1 for a split of n, best first.  Score = modelled
+   HBM passes: the column pass costs 1 + seg_pen(lines per tile x element),
+   a single-pass second factor 1 + seg_pen of its transposed store, a
+   further split about 2.3; balance breaks ties.  Returns the number of
+   candidates (<= max). */
 static int split_candidates(int prec, long long n, long long *out, int max)
 {
     long long n1;
@@ -632,13 +648,20 @@ static int split_candidates(int prec, long long n, long long *out, int max)
     for (n1 = 2; n1 <= 16384 && n1 < n; ++n1) {
         long long n2;
         double score;
+        const fk_kernel *kc, *kt;
         if (n % n1) continue;
         n2 = n / n1;
-        if (!find_kernel(prec, n1, FK_COL, FK_IO_PLAIN)) continue;
+        kc = find_kernel(prec, n1, FK_COL, FK_IO_PLAIN);
+        if (!kc) continue;
         if (!smooth_plannable(prec, n2, 1)) continue;
-        score = fabs(log((double)n1) - log((double)n2));
-        if (!has_single(prec, n2)) score += 100.0;
-        if (n1 < 16) score += 10.0;
+        score = 1.0 + seg_pen((double)kc->nl * csz(prec));
+        if (has_single(prec, n2)) {
+            kt = find_kernel(prec, n2, FK_ROW, FK_IO_TOUT);
+            score += 1.0 + (kt ? seg_pen((double)kt->nl * csz(prec)) : 3.0);
+        } else {
+            score += 2.3;
+        }
+        score += 0.05 * fabs(log((double)n1) - log((double)n2));
         score -= 1e-9 * (double)n2; /* ties: larger second (row) factor */
         if (nc < 64) {
             sc[nc] = score;
@@ -700,6 +723,7 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
         return 0;
     }
     kr = find_kernel(prec, n, FK_ROW, FK_IO_PLAIN);
+    if (c->r16 && n != 16 && n != 256 && n != 4096) kr = 0;
     if (kr) {
         /* candidate thread mappings, the heuristic's choice first:
            0 ROW, 1 ROW with transposed store, 2 COL */
@@ -741,6 +765,13 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
     if (smooth_plannable(prec, n, 0)) {
         long long cands[4], n1, n2, H, s;
         int ncand = split_candidates(prec, n, cands, 4);
+        if (c->r16 && is_pow16(n)) {
+            /* matrix-core variant: radix-16 passes only (16, 256, 4096) */
+            ncand = 0;
+            if (n % 4096 == 0 && is_pow16(n / 4096)) cands[ncand++] = 4096;
+            if (n % 256 == 0 && is_pow16(n / 256)) cands[ncand++] = 256;
+            if (is_pow16(n / 16)) cands[ncand++] = 16;
+        }
         fk_dim ab[FK_MAXRANK * 2 + 1], bb[FK_MAXRANK * 2 + 1];
         fk_pass proto;
         const fk_kernel *kA;
@@ -1122,6 +1153,7 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
     c.err = 0;
     c.choice = choice;
     c.nchoice = nchoice;
+    c.r16 = (pb.flags & FK_MFMA) && pb.prec == 1 && pb.kind == FK_C2C;
     count_flops(P);
     for (i = 0; i < pb.vrank; ++i)
         if (pb.vdims[i].n == 0) goto done_ok; /* howmany = 0: nothing to do */
@@ -1203,6 +1235,7 @@ done_ok:
         goto fail;
     }
     P->nchoice = c.ci < FK_MAX_DECIDE ? c.ci : FK_MAX_DECIDE;
+    if (c.r16) fk_plan_use_mfma(P);
     if (cout) *cout = c;
     ok = 1;
 fail:
@@ -1222,7 +1255,7 @@ static unsigned long long problem_hash(const fk_problem *pb)
     v[m++] = pb->kind;
     v[m++] = pb->prec;
     v[m++] = pb->kind == FK_C2C ? pb->sign : 0;
-    v[m++] = pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT);
+    v[m++] = pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT | FK_MFMA);
     v[m++] = pb->in == pb->out;
     v[m++] = (long long)((((uintptr_t)pb->in) | ((uintptr_t)pb->out)) % 16);
     v[m++] = pb->rank;
@@ -1410,6 +1443,46 @@ int fk_execute(fk_plan *p, void *in, void *out)
     if (own) __sync_lock_release(&p->scratch_busy);
     if (rc) set_err("execute failed: %s", hipGetErrorString((hipError_t)rc));
     return rc;
+}
+
+/* Switch every FP32 pass of length 16, 256 or 4096 with all-16 radices to
+   the split-precision matrix-core kernel (fk_mfma.hip).  Returns the
+   number of passes switched. */
+int fkm_mfma_kernel(int L, fk_kernel *k);
+
+static fk_kernel g_mk[3];
+static pthread_once_t g_mk_once = PTHREAD_ONCE_INIT;
+
+static void mk_init(void)
+{
+    fkm_mfma_kernel(16, &g_mk[0]);
+    fkm_mfma_kernel(256, &g_mk[1]);
+    fkm_mfma_kernel(4096, &g_mk[2]);
+}
+
+int fk_plan_use_mfma(fk_plan *p)
+{
+    fk_kernel *mk = g_mk;
+    int i, j, n = 0;
+    if (!p || p->prec != 1) return 0;
+    pthread_once(&g_mk_once, mk_init);
+    for (i = 0; i < p->npass; ++i) {
+        fk_pass *ps = p->pass + i;
+        const fk_kernel *k = ps->k;
+        int all16 = 1;
+        if (ps->type != FK_PASS_STOCKHAM || k->io == FK_IO_R2C || k->io == FK_IO_C2R ||
+            k->io == FK_IO_BLUE)
+            continue;
+        for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
+        if (!all16 || (k->L != 16 && k->L != 256 && k->L != 4096)) continue;
+        ps->k = &mk[k->L == 16 ? 0 : k->L == 256 ? 1 : 2];
+        ps->block = (unsigned)ps->k->tpl;
+        ps->lds = 0;
+        ps->grid = (unsigned)ps->a.nlines;
+        ps->a.xcd_remap = 0;
+        ++n;
+    }
+    return n;
 }
 
 /* Enqueue every pass on `st` with the plan's own scratch; no host

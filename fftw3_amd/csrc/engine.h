@@ -38,6 +38,7 @@ enum { FK_PASS_STOCKHAM = 0, FK_PASS_EW = 1 };
 #define FK_PATIENT (1U << 5)
 #define FK_ESTIMATE (1U << 6)
 #define FK_WISDOM_ONLY (1U << 21)
+#define FK_MFMA (1U << 24)          /* project flag: split-precision radix-16 */
 
 typedef struct { long long n, is, os; } fk_dim;
 
@@ -130,6 +131,8 @@ int fk_batch_execute(fk_batch *b, int use_graph);
 int fk_batch_info(fk_batch *b, char *buf, int len);
 hipStream_t fk_batch_stream(fk_batch *b);
 void fk_batch_destroy(fk_batch *b);
+int fk_plan_use_mfma(fk_plan *p);
+int fkm_mfma_probe(float *dev_out, hipStream_t st);
 int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st);
 int fk_enqueue_bound(fk_plan *p, void *in, void *out, void *const *scratch, hipStream_t st);
 int fkm_reset_counters(unsigned long long *c, int n, hipStream_t st);

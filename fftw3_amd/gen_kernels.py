@@ -123,6 +123,9 @@ def variants(prec, L, rads, mode, io, g0):
         return []
     out, seen = [], {(tuple(rads), g0["tpl"], g0["nl"])}
     cands = [(rads, 2, 1.0), (rads, 1, 2.0), (rads, 1, 0.5)]
+    if io == "TOUT":
+        # transposed stores want many lines per tile (long output runs)
+        cands += [(rads, 1, 4.0), (rads, 2, 4.0), (rads, 2, 8.0)]
     if list(reversed(rads)) != list(rads):
         cands.append((list(reversed(rads)), 1, 1.0))
     for r, td, nm in cands:

@@ -60,6 +60,18 @@ double fftw_amd_plan_pass_bytes(const void *plan, int i);
    whole execute.  Negative on error. */
 double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *per_pass_ms);
 
+/* Split-precision matrix-core variant (FP32 plans only): every pass of
+   length 16, 256 or 4096 with radix-16 stages runs its butterflies as FP16
+   hi/lo matrix products with FP32 accumulation (v_mfma_f32_32x32x8_f16).
+   Returns the number of passes switched (0 for other plans). */
+int fftw_amd_plan_use_mfma(void *fftwf_plan);
+/* Planner flag for fftwf_ c2c plans: decompose power-of-16 lengths into
+   16 / 256 / 4096-point passes and run them on the matrix-core variant. */
+#define FFTW_AMD_MFMA (1U << 24)
+/* Writes the 32x32 result of one probe matrix product (identity rows 0..7
+   times a known B) to dev_out, for checking the operand layouts. */
+int fftw_amd_mfma_probe(float *dev_out);
+
 /* Batched transforms of uneven lengths.  Group g holds howmany[g]
    contiguous 1-D c2c transforms of length n[g] (distance n[g]) from in[g] to
    out[g] (device memory).  Every group is planned with `flags` (FFTW_MEASURE
