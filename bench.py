@@ -426,6 +426,24 @@ def main():
         log("calibration copy: 2 dispatches of %d bytes read + %d bytes written" % (tot3 * 16, tot3 * 16))
         del tmp
 
+    # ---- streaming-copy bandwidth of this device (hipMemcpy device to
+    # device, 1 GiB read + 1 GiB written), the practical HBM ceiling ----
+    copy_gbs = None
+    if cmid.nbytes >= (1 << 30) and rout.nbytes >= (1 << 30):
+        c0, c1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
+        best = None
+        for _ in range(4):
+            L.fftw_amd_event_record(c0, None)
+            L.fftw_amd_memcpy_dtod(rout.ptr, cmid.ptr, 1 << 30)
+            L.fftw_amd_event_record(c1, None)
+            L.fftw_amd_synchronize()
+            t = L.fftw_amd_event_elapsed(c0, c1)
+            best = t if best is None else min(best, t)
+        copy_gbs = 2.0 * (1 << 30) / (best * 1e-3) / 1e9
+        L.fftw_amd_event_destroy(c0)
+        L.fftw_amd_event_destroy(c1)
+        log("device copy: %.1f GB/s (1 GiB hipMemcpy, best of 4)" % copy_gbs)
+
     def step():
         p3f.execute()
         p3b.execute()
@@ -507,6 +525,8 @@ def main():
             "peak": PEAK_HBM,
             "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM, 4),
+            "device_copy_GBps": None if copy_gbs is None else round(copy_gbs, 1),
+            "frac_of_copy": None if copy_gbs is None else round(achieved / copy_gbs, 4),
             "traffic": traffic,
             "algorithmic_bytes_per_launch": tby / cnt,
             "mean_launch_ms": round(tms / cnt, 4),
@@ -524,10 +544,14 @@ def kernel_symbol(info):
         return info
     kv = dict(x.split("=", 1) for x in f if "=" in x)
     rads = kv["rad"].split("x")
+    real = "double" if f[1] == "f64" else "float"
+    mode = 0 if f[4] == "row" else 1
+    if f[5] == "loop":
+        return "void fk::stockham_loop<%s, %s, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
+            real, kv["nl"], mode, rads[0], ", ".join(rads), kv["tpl"])
     io = {"plain": 0, "tout": 1, "r2c": 2, "c2r": 3, "bluestein": 4}[f[5]]
     return "void fk::stockham<%s, %s, %d, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
-        "double" if f[1] == "f64" else "float", kv["nl"], 0 if f[4] == "row" else 1, io, rads[0],
-        ", ".join(rads), kv["tpl"])
+        real, kv["nl"], mode, io, rads[0], ", ".join(rads), kv["tpl"])
 
 
 def committed_traffic(symbol):

@@ -155,6 +155,17 @@ def lib():
         f("plan_guru64_dft_r2c", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, u])
         f("plan_guru64_dft_c2r", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, u])
         f("plan_guru_split_dft", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, vp, vp, u])
+        f("plan_guru64_split_dft", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, vp, vp,
+                                        u])
+        f("plan_guru_split_dft_r2c", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, vp, u])
+        f("plan_guru_split_dft_c2r", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, vp, u])
+        f("plan_guru64_split_dft_r2c", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, vp,
+                                            u])
+        f("plan_guru64_split_dft_c2r", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, vp,
+                                            u])
+        f("execute_split_dft", None, [vp, vp, vp, vp, vp])
+        f("execute_split_dft_r2c", None, [vp, vp, vp, vp])
+        f("execute_split_dft_c2r", None, [vp, vp, vp, vp])
         f("plan_many_r2r", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, vp, u])
         f("execute", None, [vp])
         f("execute_dft", None, [vp, vp, vp])
@@ -332,6 +343,15 @@ class Plan:
     def execute_c2r(self, i, o):
         fn(self.prec, "execute_dft_c2r")(self.h, i, o)
 
+    def execute_split(self, ri, ii, ro, io):
+        fn(self.prec, "execute_split_dft")(self.h, ri, ii, ro, io)
+
+    def execute_split_r2c(self, i, ro, io):
+        fn(self.prec, "execute_split_dft_r2c")(self.h, i, ro, io)
+
+    def execute_split_c2r(self, ri, ii, o):
+        fn(self.prec, "execute_split_dft_c2r")(self.h, ri, ii, o)
+
     def npasses(self):
         return lib().fftw_amd_plan_npasses(self.h)
 
@@ -465,6 +485,26 @@ def plan_guru_dft_r2c(prec, dims, hdims, inp, out, flags=FFTW_ESTIMATE, wide=Fal
 def plan_guru_dft_c2r(prec, dims, hdims, inp, out, flags=FFTW_ESTIMATE, wide=False):
     name = "plan_guru64_dft_c2r" if wide else "plan_guru_dft_c2r"
     h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, out, flags)
+    return Plan(h, prec)
+
+
+def plan_guru_split_dft(prec, dims, hdims, ri, ii, ro, io, flags=FFTW_ESTIMATE, wide=False):
+    """Split-complex guru c2c; strides count reals.  Forward on (ri, ii);
+    pass (ii, ri, io, ro) for the backward transform."""
+    name = "plan_guru64_split_dft" if wide else "plan_guru_split_dft"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), ri, ii, ro, io, flags)
+    return Plan(h, prec)
+
+
+def plan_guru_split_dft_r2c(prec, dims, hdims, inp, ro, io, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_split_dft_r2c" if wide else "plan_guru_split_dft_r2c"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, ro, io, flags)
+    return Plan(h, prec)
+
+
+def plan_guru_split_dft_c2r(prec, dims, hdims, ri, ii, out, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_split_dft_c2r" if wide else "plan_guru_split_dft_c2r"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), ri, ii, out, flags)
     return Plan(h, prec)
 
 

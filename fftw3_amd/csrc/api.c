@@ -15,6 +15,7 @@
  * not needed because the kernels address complex elements directly.
  */
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "engine.h"
@@ -162,23 +163,81 @@ PLAN X(plan_guru64_dft)(int rank, const X(iodim64) *dims, int howmany_rank,
     return mk(&pb);
 }
 
-/* split-complex arrays are not supported by this engine */
+/* ---------------- split-complex arrays ----------------
+   api/plan-guru-split-dft.c: real and imaginary parts in separate arrays,
+   strides counting reals, no sign argument (forward on (ri, ii) -> (ro, io);
+   the backward transform is obtained by swapping the pointers of each pair,
+   doc/reference.texi "Guru Complex DFTs").  A pair that is one interleaved
+   array (im = re + 1, or re = im + 1 for the swapped form, every stride
+   even) runs on the interleaved kernels directly; separate arrays are
+   staged through an interleaved buffer by the engine. */
+
+/* side 0: input strides, 1: output strides */
+static int split_pairing(const R *re, const R *im, const fk_problem *pb, int side)
+{
+    int i;
+    for (i = 0; i < pb->rank; ++i)
+        if ((side ? pb->dims[i].os : pb->dims[i].is) % 2) return FK_SPLIT_NONE;
+    for (i = 0; i < pb->vrank; ++i)
+        if ((side ? pb->vdims[i].os : pb->vdims[i].is) % 2) return FK_SPLIT_NONE;
+    if (im == re + 1 && ((uintptr_t)re % sizeof(C)) == 0) return FK_SPLIT_PAIRED;
+    if (re == im + 1 && ((uintptr_t)im % sizeof(C)) == 0) return FK_SPLIT_SWAPPED;
+    return FK_SPLIT_NONE;
+}
+
+static void halve_strides(fk_problem *pb, int side)
+{
+    int i;
+    for (i = 0; i < pb->rank; ++i) {
+        if (side) pb->dims[i].os /= 2;
+        else pb->dims[i].is /= 2;
+    }
+    for (i = 0; i < pb->vrank; ++i) {
+        if (side) pb->vdims[i].os /= 2;
+        else pb->vdims[i].is /= 2;
+    }
+}
+
+static PLAN mk_split_c2c(fk_problem *pb, R *ri, R *ii, R *ro, R *io)
+{
+    int mi = split_pairing(ri, ii, pb, 0), mo = split_pairing(ro, io, pb, 1);
+    pb->kind = FK_C2C;
+    if (mi != FK_SPLIT_NONE && mi == mo) {
+        fk_plan *P;
+        halve_strides(pb, 0);
+        halve_strides(pb, 1);
+        pb->sign = mi == FK_SPLIT_PAIRED ? -1 : 1;
+        pb->in = mi == FK_SPLIT_PAIRED ? (void *)ri : (void *)ii;
+        pb->out = mo == FK_SPLIT_PAIRED ? (void *)ro : (void *)io;
+        P = (fk_plan *)mk(pb);
+        if (P) P->split = mi;
+        return (PLAN)P;
+    }
+    pb->sign = -1;
+    pb->split = FK_SPLIT_GENERAL;
+    pb->in = ri;
+    pb->in2 = ii;
+    pb->out = ro;
+    pb->out2 = io;
+    return mk(pb);
+}
+
 PLAN X(plan_guru_split_dft)(int rank, const X(iodim) *dims, int howmany_rank,
                             const X(iodim) *howmany_dims, R *ri, R *ii, R *ro, R *io,
                             unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)ri; (void)ii; (void)ro; (void)io; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    return mk_split_c2c(&pb, ri, ii, ro, io);
 }
 
 PLAN X(plan_guru64_split_dft)(int rank, const X(iodim64) *dims, int howmany_rank,
                               const X(iodim64) *howmany_dims, R *ri, R *ii, R *ro, R *io,
                               unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)ri; (void)ii; (void)ro; (void)io; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    return mk_split_c2c(&pb, ri, ii, ro, io);
 }
 
 static fk_plan *P_(PLAN p) { return (fk_plan *)p; }
@@ -196,9 +255,35 @@ void X(execute_dft)(const PLAN p, C *in, C *out)
     fk_execute_or_die(P_(p), in, out);
 }
 
+static void die(const char *msg)
+{
+    fprintf(stderr, "fftw3_amd: %s\n", msg);
+    abort();
+}
+
+static void exec4_or_die(fk_plan *P, void *in, void *in2, void *out, void *out2)
+{
+    if (fk_execute4(P, in, in2, out, out2)) die(fk_last_error());
+}
+
+/* api/execute-split-dft.c:25-29: new arrays of the same layout.  A plan
+   that runs on interleaved kernels needs the same pairing as at planning. */
 void X(execute_split_dft)(const PLAN p, R *ri, R *ii, R *ro, R *io)
 {
-    (void)p; (void)ri; (void)ii; (void)ro; (void)io;
+    fk_plan *P = P_(p);
+    if (!P) return;
+    if (P->split == FK_SPLIT_GENERAL) {
+        exec4_or_die(P, ri, ii, ro, io);
+    } else if (P->split == FK_SPLIT_SWAPPED) {
+        if (ri != ii + 1 || ro != io + 1)
+            die("execute_split_dft: arrays must keep the pairing of the planning arrays");
+        fk_execute_or_die(P, ii, io);
+    } else {
+        /* paired split plans, and interleaved plans given split views */
+        if (ii != ri + 1 || io != ro + 1)
+            die("execute_split_dft: arrays must keep the pairing of the planning arrays");
+        fk_execute_or_die(P, ri, ro);
+    }
 }
 
 /* ---------------- real DFT ---------------- */
@@ -364,40 +449,80 @@ PLAN X(plan_guru64_dft_c2r)(int rank, const X(iodim64) *dims, int howmany_rank,
     return mk(&pb);
 }
 
+static PLAN mk_split_r2c(fk_problem *pb, R *in, R *ro, R *io)
+{
+    pb->kind = FK_R2C;
+    pb->sign = -1;
+    pb->in = in;
+    if (split_pairing(ro, io, pb, 1) == FK_SPLIT_PAIRED) {
+        fk_plan *P;
+        halve_strides(pb, 1);
+        pb->out = ro;
+        P = (fk_plan *)mk(pb);
+        if (P) P->split = FK_SPLIT_PAIRED;
+        return (PLAN)P;
+    }
+    pb->split = FK_SPLIT_GENERAL;
+    pb->out = ro;
+    pb->out2 = io;
+    return mk(pb);
+}
+
+static PLAN mk_split_c2r(fk_problem *pb, R *ri, R *ii, R *out)
+{
+    pb->kind = FK_C2R;
+    pb->sign = 1;
+    pb->out = out;
+    if (split_pairing(ri, ii, pb, 0) == FK_SPLIT_PAIRED) {
+        fk_plan *P;
+        halve_strides(pb, 0);
+        pb->in = ri;
+        if ((void *)ri != (void *)out) pb->flags |= FFTW_DESTROY_INPUT;
+        P = (fk_plan *)mk(pb);
+        if (P) P->split = FK_SPLIT_PAIRED;
+        return (PLAN)P;
+    }
+    /* staged: the caller's arrays are only read */
+    pb->split = FK_SPLIT_GENERAL;
+    pb->in = ri;
+    pb->in2 = ii;
+    return mk(pb);
+}
+
 PLAN X(plan_guru_split_dft_r2c)(int rank, const X(iodim) *dims, int howmany_rank,
                                 const X(iodim) *howmany_dims, R *in, R *ro, R *io,
                                 unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)in; (void)ro; (void)io; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    return mk_split_r2c(&pb, in, ro, io);
 }
 
 PLAN X(plan_guru_split_dft_c2r)(int rank, const X(iodim) *dims, int howmany_rank,
                                 const X(iodim) *howmany_dims, R *ri, R *ii, R *out,
                                 unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)ri; (void)ii; (void)out; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    return mk_split_c2r(&pb, ri, ii, out);
 }
 
 PLAN X(plan_guru64_split_dft_r2c)(int rank, const X(iodim64) *dims, int howmany_rank,
                                   const X(iodim64) *howmany_dims, R *in, R *ro, R *io,
                                   unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)in; (void)ro; (void)io; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    return mk_split_r2c(&pb, in, ro, io);
 }
 
 PLAN X(plan_guru64_split_dft_c2r)(int rank, const X(iodim64) *dims, int howmany_rank,
                                   const X(iodim64) *howmany_dims, R *ri, R *ii, R *out,
                                   unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims;
-    (void)ri; (void)ii; (void)out; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    return mk_split_c2r(&pb, ri, ii, out);
 }
 
 /* api/execute-dft-r2c.c:25-30 / execute-dft-c2r.c:25-30 */
@@ -415,12 +540,28 @@ void X(execute_dft_c2r)(const PLAN p, C *in, R *out)
 
 void X(execute_split_dft_r2c)(const PLAN p, R *in, R *ro, R *io)
 {
-    (void)p; (void)in; (void)ro; (void)io;
+    fk_plan *P = P_(p);
+    if (!P) return;
+    if (P->split == FK_SPLIT_GENERAL) {
+        exec4_or_die(P, in, 0, ro, io);
+    } else {
+        if (io != ro + 1)
+            die("execute_split_dft_r2c: arrays must keep the pairing of the planning arrays");
+        fk_execute_or_die(P, in, ro);
+    }
 }
 
 void X(execute_split_dft_c2r)(const PLAN p, R *ri, R *ii, R *out)
 {
-    (void)p; (void)ri; (void)ii; (void)out;
+    fk_plan *P = P_(p);
+    if (!P) return;
+    if (P->split == FK_SPLIT_GENERAL) {
+        exec4_or_die(P, ri, ii, out, 0);
+    } else {
+        if (ii != ri + 1)
+            die("execute_split_dft_c2r: arrays must keep the pairing of the planning arrays");
+        fk_execute_or_die(P, ri, out);
+    }
 }
 
 /* ---------------- r2r (not provided: planners return NULL) ---------------- */

@@ -185,7 +185,9 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
                 const fk_pass *ps = b->plans[g]->pass + i;
                 if (ps->type != FK_PASS_STOCKHAM || fk_menu_kind(menu, nm, ps->k) < 0)
                     b->persist[g] = 0;
-                if (ps->type != FK_PASS_STOCKHAM || ps->grid < 4u * (unsigned)ncu) big = 0;
+                if (ps->type != FK_PASS_STOCKHAM ||
+                    (ps->a.nlines + ps->k->nl - 1) / ps->k->nl < 4LL * ncu)
+                    big = 0;
             }
             if (big) b->persist[g] = 0;
         }
@@ -193,10 +195,11 @@ fk_batch *fk_batch_create(int prec, int ngroups, const int *n, const int *howman
     /* work tables, one per phase */
     for (g = 0; g < ngroups; ++g) {
         fk_plan *P = b->plans[g];
-        void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+        void *buf[FK_NBUF];
         if (!b->persist[g]) continue;
         buf[FK_BUF_IN] = in[g];
         buf[FK_BUF_OUT] = out[g];
+        buf[FK_BUF_IN2] = buf[FK_BUF_OUT2] = 0;
         for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = P->scratch[i];
         for (i = 0; i < P->npass; ++i) {
             const fk_pass *ps = P->pass + i;

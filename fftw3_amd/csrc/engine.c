@@ -38,6 +38,8 @@ static hipStream_t g_stream = 0;
 static int g_async = 0;
 static double g_timelimit = -1.0;
 static __thread char g_err[512];
+static int g_ncu = 256;        /* compute units of device 0 */
+static int *g_occ = 0;         /* resident workgroups per CU of each looped kernel */
 
 static void set_err(const char *fmt, ...)
 {
@@ -72,8 +74,39 @@ int fk_init(void)
             }
         }
     }
+    if (hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+        g_ncu < 1)
+        g_ncu = 256;
+    g_occ = (int *)calloc((size_t)g_nreg, sizeof(int));
+    if (!g_occ) {
+        set_err("out of host memory");
+        return -1;
+    }
+    for (i = 0; i < g_nreg; ++i) {
+        int n = 0;
+        if (!g_reg[i].loop) continue;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, g_reg[i].fn, g_reg[i].tpl * g_reg[i].nl,
+                                                         g_reg[i].lds) != hipSuccess) {
+            (void)hipGetLastError();
+            n = 1;
+        }
+        g_occ[i] = n < 1 ? 1 : n;
+    }
     g_inited = 1;
     return 0;
+}
+
+/* Grid of a looped (persistent) kernel: the workgroups resident at once,
+   rounded to a multiple of 8 (one share per XCD), no more than the tiles
+   need. */
+static unsigned loop_grid(const fk_kernel *k, long long nlines)
+{
+    long long ntiles = (nlines + k->nl - 1) / k->nl;
+    long long occ = (g_occ && k >= g_reg && k < g_reg + g_nreg) ? g_occ[k - g_reg] : 1;
+    long long g = (occ < 1 ? 1 : occ) * g_ncu, need = (ntiles + 7) / 8 * 8;
+    if (g > need) g = need;
+    g = g / 8 * 8;
+    return (unsigned)(g < 8 ? 8 : g);
 }
 
 /* ------------------------------------------------------------------ */
@@ -300,6 +333,7 @@ typedef struct {
     int nchoice;
     int ci;                     /* decisions taken so far */
     int r16;                    /* restrict to radix-16 passes (FK_MFMA) */
+    int bin, bout;              /* buffers of the r2c / c2r problem's arrays */
     int nalt[FK_MAX_DECIDE];    /* alternatives at each decision */
     int dkind[FK_MAX_DECIDE];
 } ctx;
@@ -446,18 +480,24 @@ static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b, si
     int nloop = nb > 3 ? nb - 3 : 0, i;
     for (;;) {
         long long ioff = proto->ioff, ooff = proto->ooff;
+        long long i2off = proto->i2off, o2off = proto->o2off;
         fk_pass *p;
         for (i = 0; i < nloop; ++i) {
             ioff += idx[i] * b[3 + i].is * (long long)isz;
             ooff += idx[i] * b[3 + i].os * (long long)osz;
+            i2off += idx[i] * b[3 + i].is * (long long)isz;
+            o2off += idx[i] * b[3 + i].os * (long long)osz;
         }
         p = new_pass(c);
         *p = *proto;
         p->ioff = ioff;
         p->ooff = ooff;
+        p->i2off = i2off;
+        p->o2off = o2off;
         if (p->type == FK_PASS_STOCKHAM) {
             fill_batch3(p->a.nb, p->a.ids, p->a.ods, &p->a.nlines, nb < 3 ? nb : 3, b);
-            p->grid = (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
+            p->grid = p->k->loop ? loop_grid(p->k, p->a.nlines)
+                                 : (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
             if (p->a.nlines == 0) --c->P->npass;
         } else {
             fill_batch3(p->e.nb, p->e.ids, p->e.ods, &p->e.nlines, nb < 3 ? nb : 3, b);
@@ -913,8 +953,8 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         for (i = 0; i < nb; ++i) unit &= (b[i].is % 2) == 0;
         if (unit) fl |= FK_REAL_UNIT;
         if (!wk) { c->err = 1; return -1; }
-        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, FK_BUF_IN, 0,
-                      FK_BUF_OUT, 0, fl, rs, cs);
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, c->bin, 0,
+                      c->bout, 0, fl, rs, cs);
         for (; before < (size_t)P->npass; ++before) {
             fk_pass *p = P->pass + before;
             p->a.aux = wk;
@@ -942,7 +982,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         proto.e.n = nl;
         proto.e.is = pb->dims[r - 1].is;
         proto.e.os = 1;
-        proto.ibuf = FK_BUF_IN;
+        proto.ibuf = c->bin;
         proto.obuf = FK_BUF_S0 + 6;
         proto.bytes = (double)H * nl * (rs + cs);
         emit_loops(c, &proto, nb, sb, rs, cs);
@@ -957,7 +997,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         proto.e.is = 1;
         proto.e.os = pb->dims[r - 1].os;
         proto.ibuf = FK_BUF_S0 + 7;
-        proto.obuf = FK_BUF_OUT;
+        proto.obuf = c->bout;
         proto.bytes = (double)H * nh * 2 * cs;
         emit_loops(c, &proto, nb, sb, cs, cs);
         desc_add(P, "(r2c-generic n=%lld)", nl);
@@ -979,7 +1019,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
             vd[nv].is = vd[nv].os = pb->vdims[i].os;
             ++nv;
         }
-        if (plan_c2c_dims(c, r - 1, lead, nv, vd, FK_BUF_OUT, FK_BUF_OUT, 0)) return -1;
+        if (plan_c2c_dims(c, r - 1, lead, nv, vd, c->bout, c->bout, 0)) return -1;
     }
     return 0;
 }
@@ -1010,7 +1050,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
             vd[nv].is = vd[nv].os = pb->vdims[i].is;
             ++nv;
         }
-        if (plan_c2c_dims(c, r - 1, lead, nv, vd, FK_BUF_IN, FK_BUF_IN, 1)) return -1;
+        if (plan_c2c_dims(c, r - 1, lead, nv, vd, c->bin, c->bin, 1)) return -1;
     }
     for (i = 0; i < r - 1; ++i) b[nb++] = pb->dims[i];
     for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
@@ -1025,8 +1065,8 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         for (i = 0; i < nb; ++i) unit &= (b[i].os % 2) == 0;
         if (unit) fl |= FK_REAL_UNIT;
         if (!wk) { c->err = 1; return -1; }
-        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, FK_BUF_IN, 0,
-                      FK_BUF_OUT, 0, fl, cs, rs);
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, c->bin, 0,
+                      c->bout, 0, fl, cs, rs);
         for (; before < (size_t)P->npass; ++before) {
             fk_pass *p = P->pass + before;
             p->a.aux = wk;
@@ -1055,7 +1095,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         proto.e.p0 = nl;
         proto.e.is = pb->dims[r - 1].is;
         proto.e.os = 1;
-        proto.ibuf = FK_BUF_IN;
+        proto.ibuf = c->bin;
         proto.obuf = FK_BUF_S0 + 6;
         proto.bytes = (double)H * (nh + nl) * cs;
         emit_loops(c, &proto, nb, sb, cs, cs);
@@ -1070,7 +1110,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         proto.e.is = 1;
         proto.e.os = pb->dims[r - 1].os;
         proto.ibuf = FK_BUF_S0 + 7;
-        proto.obuf = FK_BUF_OUT;
+        proto.obuf = c->bout;
         proto.bytes = (double)H * nl * (cs + rs);
         emit_loops(c, &proto, nb, sb, cs, rs);
         desc_add(P, "(c2r-generic n=%lld)", nl);
@@ -1121,6 +1161,127 @@ static void count_flops(fk_plan *P)
     }
 }
 
+/* Contiguous row-major layout of a complex tensor: dims (n of the complex
+   side), then vdims outermost; writes strides into sd / sv, returns the
+   element count. */
+static long long contig_layout(int rank, const fk_dim *dims, int vrank, const fk_dim *vdims,
+                               long long nlast, fk_dim *sd, fk_dim *sv)
+{
+    long long s = 1;
+    int i;
+    for (i = rank - 1; i >= 0; --i) {
+        sd[i].n = i == rank - 1 ? nlast : dims[i].n;
+        sd[i].is = sd[i].os = s;
+        s *= sd[i].n;
+    }
+    for (i = vrank - 1; i >= 0; --i) {
+        sv[i].n = vdims[i].n;
+        sv[i].is = sv[i].os = s;
+        s *= sv[i].n;
+    }
+    return s;
+}
+
+/* Elementwise pass between the split arrays of the user (strides in reals,
+   from dims / vdims) and the interleaved staging buffer (layout sd / sv):
+   gather = split -> staging, scatter = staging -> split. */
+static void emit_split_ew(ctx *c, int gather, int rank, const fk_dim *dims, int vrank,
+                          const fk_dim *vdims, const fk_dim *sd, const fk_dim *sv)
+{
+    fk_pass proto;
+    fk_dim b[FK_MAXRANK * 2];
+    int nb = 0, i, prec = c->P->prec;
+    size_t cs = csz(prec), rs = rsz(prec);
+    long long cnt = 1;
+    memset(&proto, 0, sizeof(proto));
+    proto.type = FK_PASS_EW;
+    proto.e.op = gather ? FK_EW_SPLIT_GATHER : FK_EW_SPLIT_SCATTER;
+    if (rank > 0) {
+        proto.e.n = sd[rank - 1].n;
+        proto.e.is = gather ? dims[rank - 1].is : 1;
+        proto.e.os = gather ? 1 : dims[rank - 1].os;
+    } else {
+        proto.e.n = 1;
+    }
+    for (i = 0; i < rank - 1; ++i, ++nb) {
+        b[nb].n = sd[i].n;
+        b[nb].is = gather ? dims[i].is : sd[i].is;
+        b[nb].os = gather ? sd[i].os : dims[i].os;
+    }
+    for (i = 0; i < vrank; ++i, ++nb) {
+        b[nb].n = sv[i].n;
+        b[nb].is = gather ? vdims[i].is : sv[i].is;
+        b[nb].os = gather ? sv[i].os : vdims[i].os;
+    }
+    for (i = 0; i < nb; ++i) cnt *= b[i].n;
+    cnt *= proto.e.n;
+    if (gather) {
+        proto.ibuf = FK_BUF_IN;
+        proto.i2buf = FK_BUF_IN2;
+        proto.obuf = FK_BUF_S0 + FK_SPLIT_SLOT;
+    } else {
+        proto.ibuf = FK_BUF_S0 + FK_SPLIT_SLOT;
+        proto.obuf = FK_BUF_OUT;
+        proto.o2buf = FK_BUF_OUT2;
+    }
+    proto.bytes = (double)cnt * (double)(cs + 2 * rs);
+    emit_loops(c, &proto, nb, b, gather ? rs : cs, gather ? cs : rs);
+}
+
+/* Split-complex problems with separate real and imaginary arrays
+   (api/plan-guru-split-dft.c, api/plan-guru-split-dft-r2c.c,
+   api/plan-guru-split-dft-c2r.c): the complex side is staged through an
+   interleaved buffer, so every split plan is gather -> interleaved plan ->
+   scatter (c2c), interleaved r2c -> scatter, or gather -> c2r.  The
+   transform is the forward one on (ri, ii); a backward transform is the
+   caller swapping the real and imaginary pointers, as in FFTW. */
+static int plan_split(ctx *c, const fk_problem *pb)
+{
+    fk_plan *P = c->P;
+    int r = pb->rank, i, prec = P->prec;
+    fk_dim sd[FK_MAXRANK], sv[FK_MAXRANK];
+    long long nlast = r > 0 ? pb->dims[r - 1].n : 1, cnt;
+    const int S = FK_BUF_S0 + FK_SPLIT_SLOT;
+    if (pb->kind != FK_C2C && r == 0) {
+        set_err("rank-0 split real transforms are not supported");
+        return -1;
+    }
+    if (pb->kind != FK_C2C) nlast = nlast / 2 + 1;
+    cnt = contig_layout(r, pb->dims, pb->vrank, pb->vdims, nlast, sd, sv);
+    need_scratch(c, FK_SPLIT_SLOT, csz(prec) * (size_t)cnt);
+    if (pb->kind == FK_C2C) {
+        desc_add(P, "(dft-split rank=%d", r);
+        for (i = 0; i < r; ++i) desc_add(P, " %lld", pb->dims[i].n);
+        desc_add(P, ")");
+        emit_split_ew(c, 1, r, pb->dims, pb->vrank, pb->vdims, sd, sv);
+        if (r > 0 && plan_c2c_dims(c, r, sd, pb->vrank, sv, S, S, 0)) return -1;
+        emit_split_ew(c, 0, r, pb->dims, pb->vrank, pb->vdims, sd, sv);
+    } else if (pb->kind == FK_R2C) {
+        fk_problem q = *pb;
+        for (i = 0; i < r; ++i) q.dims[i].os = sd[i].os;
+        for (i = 0; i < pb->vrank; ++i) q.vdims[i].os = sv[i].os;
+        q.split = FK_SPLIT_NONE;
+        c->bout = S;
+        desc_add(P, "(rdft2-split r2c)");
+        if (plan_r2c(c, &q)) return -1;
+        c->bout = FK_BUF_OUT;
+        emit_split_ew(c, 0, r, pb->dims, pb->vrank, pb->vdims, sd, sv);
+    } else {
+        fk_problem q = *pb;
+        /* the staged copy is the c2r input, so the caller's arrays are
+           preserved */
+        for (i = 0; i < r; ++i) q.dims[i].is = sd[i].is;
+        for (i = 0; i < pb->vrank; ++i) q.vdims[i].is = sv[i].is;
+        q.split = FK_SPLIT_NONE;
+        emit_split_ew(c, 1, r, pb->dims, pb->vrank, pb->vdims, sd, sv);
+        c->bin = S;
+        desc_add(P, "(rdft2-split c2r)");
+        if (plan_c2r(c, &q)) return -1;
+        c->bin = FK_BUF_IN;
+    }
+    return c->err ? -1 : 0;
+}
+
 static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *choice, int nchoice,
                            ctx *cout)
 {
@@ -1154,10 +1315,18 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
     c.choice = choice;
     c.nchoice = nchoice;
     c.r16 = (pb.flags & FK_MFMA) && pb.prec == 1 && pb.kind == FK_C2C;
+    c.bin = FK_BUF_IN;
+    c.bout = FK_BUF_OUT;
+    P->split = pb.split;
+    P->in2 = pb.in2;
+    P->out2 = pb.out2;
+    if (pb.split == FK_SPLIT_GENERAL) P->align_need = (unsigned)rsz(pb.prec);
     count_flops(P);
     for (i = 0; i < pb.vrank; ++i)
         if (pb.vdims[i].n == 0) goto done_ok; /* howmany = 0: nothing to do */
-    if (pb.kind == FK_C2C) {
+    if (pb.split == FK_SPLIT_GENERAL) {
+        if (plan_split(&c, &pb)) goto fail;
+    } else if (pb.kind == FK_C2C) {
         int sw = pb.sign != -1;
         P->inplace = pb.in == pb.out;
         if (P->inplace && !inplace_ok(&pb)) goto fail;
@@ -1255,7 +1424,8 @@ static unsigned long long problem_hash(const fk_problem *pb)
     v[m++] = pb->kind;
     v[m++] = pb->prec;
     v[m++] = pb->kind == FK_C2C ? pb->sign : 0;
-    v[m++] = pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT | FK_MFMA);
+    v[m++] = (pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT | FK_MFMA)) |
+             ((long long)pb->split << 40);
     v[m++] = pb->in == pb->out;
     v[m++] = (long long)((((uintptr_t)pb->in) | ((uintptr_t)pb->out)) % 16);
     v[m++] = pb->rank;
@@ -1405,22 +1575,33 @@ static int launch_pass(const fk_pass *ps, void *const *buf, hipStream_t st, int 
         fk_ew_args e = ps->e;
         e.in = (const char *)buf[ps->ibuf] + ps->ioff;
         e.out = (char *)buf[ps->obuf] + ps->ooff;
+        if (e.op == FK_EW_SPLIT_GATHER) e.in2 = (const char *)buf[ps->i2buf] + ps->i2off;
+        if (e.op == FK_EW_SPLIT_SCATTER) e.out2 = (char *)buf[ps->o2buf] + ps->o2off;
         return fkm_ew(prec, &e, st);
     }
 }
 
-int fk_execute(fk_plan *p, void *in, void *out)
+int fk_execute(fk_plan *p, void *in, void *out) { return fk_execute4(p, in, 0, out, 0); }
+
+int fk_execute4(fk_plan *p, void *in, void *in2, void *out, void *out2)
 {
-    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    void *buf[FK_NBUF];
     void *tmp[FK_MAX_SCRATCH];
     int i, rc = 0, own = 0;
     hipStream_t st = p->stream;
-    if (((uintptr_t)in | (uintptr_t)out) % p->align_need) {
+    if (((uintptr_t)in | (uintptr_t)out | (uintptr_t)in2 | (uintptr_t)out2) % p->align_need) {
         set_err("execute: arrays not aligned as at planning (%u bytes)", p->align_need);
+        return (int)hipErrorInvalidValue;
+    }
+    if (p->split == FK_SPLIT_GENERAL && p->npass &&
+        ((p->kind != FK_R2C && !in2) || (p->kind != FK_C2R && !out2))) {
+        set_err("execute: split-array plan needs both real and imaginary arrays");
         return (int)hipErrorInvalidValue;
     }
     buf[FK_BUF_IN] = in;
     buf[FK_BUF_OUT] = out;
+    buf[FK_BUF_IN2] = in2;
+    buf[FK_BUF_OUT2] = out2;
     /* concurrent executes of one plan each get their own scratch */
     if (__sync_bool_compare_and_swap(&p->scratch_busy, 0, 1)) {
         own = 1;
@@ -1496,10 +1677,12 @@ int fk_enqueue(fk_plan *p, void *in, void *out, hipStream_t st)
    several executions of one plan may then overlap on the device. */
 int fk_enqueue_bound(fk_plan *p, void *in, void *out, void *const *scratch, hipStream_t st)
 {
-    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    void *buf[FK_NBUF];
     int i, rc = 0;
     buf[FK_BUF_IN] = in;
     buf[FK_BUF_OUT] = out;
+    buf[FK_BUF_IN2] = p->in2;
+    buf[FK_BUF_OUT2] = p->out2;
     for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = scratch[i];
     for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
     return rc;
@@ -1516,7 +1699,7 @@ void fk_execute_or_die(fk_plan *p, void *in, void *out)
 /* Per-pass timing with HIP events on the plan's stream. */
 double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass_ms)
 {
-    void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+    void *buf[FK_NBUF];
     hipEvent_t *ev;
     double total = 0;
     int i, it, rc = 0;
@@ -1524,6 +1707,8 @@ double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass
     if (iters < 1) iters = 1;
     buf[FK_BUF_IN] = in;
     buf[FK_BUF_OUT] = out;
+    buf[FK_BUF_IN2] = p->in2;
+    buf[FK_BUF_OUT2] = p->out2;
     for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
     ev = (hipEvent_t *)calloc((size_t)p->npass + 1, sizeof(hipEvent_t));
     for (i = 0; i <= p->npass; ++i)
@@ -1587,7 +1772,8 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
                         "stockham %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
                         "is=%lld os=%lld flags=%u",
                         ps->k->prec ? "f32" : "f64", ps->k->L, rad,
-                        ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
+                        ps->k->mode == FK_ROW ? "row" : "col",
+                        ps->k->loop ? "loop" : io_name(ps->k->io),
                         ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
                         ps->a.os, ps->a.flags);
     }

@@ -22,10 +22,21 @@ extern "C" {
 #endif
 
 #define FK_MAXRANK 16
-#define FK_MAX_SCRATCH 8
+#define FK_MAX_SCRATCH 9
+#define FK_SPLIT_SLOT 8         /* interleaved staging of split-array plans */
 
 enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2 };
-enum { FK_BUF_IN = 0, FK_BUF_OUT = 1, FK_BUF_S0 = 2 };
+/* symbolic buffers of a pass: the arrays given to execute (IN2 / OUT2 are
+   the imaginary arrays of split-complex plans) and the plan's scratch */
+enum { FK_BUF_IN = 0, FK_BUF_OUT = 1, FK_BUF_IN2 = 2, FK_BUF_OUT2 = 3, FK_BUF_S0 = 4 };
+#define FK_NBUF (FK_BUF_S0 + FK_MAX_SCRATCH)
+
+/* split-complex modes of a plan (api/plan-guru-split-dft.c: no sign
+   argument; the transform is forward on (ri, ii) -> (ro, io)) */
+enum { FK_SPLIT_NONE = 0,     /* interleaved arrays */
+       FK_SPLIT_PAIRED = 1,   /* ii = ri + 1, io = ro + 1: run as interleaved */
+       FK_SPLIT_SWAPPED = 2,  /* ri = ii + 1, ro = io + 1: interleaved backward */
+       FK_SPLIT_GENERAL = 3 };/* separate arrays: gather / transform / scatter */
 enum { FK_PASS_STOCKHAM = 0, FK_PASS_EW = 1 };
 
 /* planner flags, same values as fftw3.h (api/fftw3.h:495-519) */
@@ -49,6 +60,8 @@ typedef struct {
     fk_ew_args e;
     int ibuf, obuf;
     long long ioff, ooff;       /* byte offsets from the bound buffers */
+    int i2buf, o2buf;           /* second input / output (split ops), -1 none */
+    long long i2off, o2off;
     unsigned grid, block, lds;
     double bytes;               /* HBM bytes the pass moves (estimate) */
 } fk_pass;
@@ -60,6 +73,8 @@ typedef struct fk_plan {
     fk_dim dims[FK_MAXRANK], vdims[FK_MAXRANK];
     int inplace;
     void *in, *out;             /* arrays given at planning */
+    int split;                  /* FK_SPLIT_* */
+    void *in2, *out2;           /* imaginary arrays at planning (FK_SPLIT_GENERAL) */
     int npass, cpass;
     fk_pass *pass;
     int nscratch;
@@ -85,6 +100,10 @@ typedef struct {
     int rank, vrank;
     fk_dim dims[FK_MAXRANK], vdims[FK_MAXRANK];
     void *in, *out;
+    /* split-complex arrays (FK_SPLIT_GENERAL): strides of every dim count
+       reals; in2 / out2 are the imaginary arrays of the complex side(s) */
+    int split;
+    void *in2, *out2;
 } fk_problem;
 
 /* ---- engine entry points ---- */
@@ -92,6 +111,9 @@ int fk_init(void);
 fk_plan *fk_plan_create(const fk_problem *pb);
 void fk_plan_destroy(fk_plan *p);
 int fk_execute(fk_plan *p, void *in, void *out);         /* 0 or hip error */
+/* split-complex arrays of an FK_SPLIT_GENERAL plan (in2 / out2 unused
+   otherwise) */
+int fk_execute4(fk_plan *p, void *in, void *in2, void *out, void *out2);
 void fk_execute_or_die(fk_plan *p, void *in, void *out);
 void fk_cleanup(void);
 void fk_forget_wisdom(void);

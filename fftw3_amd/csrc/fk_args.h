@@ -65,6 +65,8 @@ typedef struct {
     int twsz;          /* complex entries in the per-stage twiddle table */
     const void *fn;    /* kernel host stub for hipLaunchKernel */
     int variant;       /* 0 = default geometry; >0 autotuner alternatives */
+    int loop;          /* 1: persistent software-pipelined form (grid = resident
+                          workgroups, a multiple of 8, each looping over tiles) */
 } fk_kernel;
 
 /* elementwise passes around the Stockham kernels (multi-pass paths) */
@@ -79,7 +81,9 @@ enum {
     FK_EW_C2R_OUT,      /* z (p0 complex) -> interleaved real output */
     FK_EW_WIDEN,        /* real -> complex (re, 0) */
     FK_EW_HERM,         /* n/2+1 bins -> full Hermitian line of n = p0 */
-    FK_EW_REAL          /* complex -> real part */
+    FK_EW_REAL,         /* complex -> real part */
+    FK_EW_SPLIT_GATHER, /* (in[j], in2[j]) reals -> interleaved complex */
+    FK_EW_SPLIT_SCATTER /* interleaved complex -> out[j] (re), out2[j] (im) */
 };
 
 typedef struct {
@@ -95,6 +99,8 @@ typedef struct {
     double scale;
     unsigned flags;           /* FK_SWAP_IN / FK_SWAP_OUT / FK_SCALE */
     int op;
+    const void *in2;          /* FK_EW_SPLIT_GATHER: imaginary input */
+    void *out2;               /* FK_EW_SPLIT_SCATTER: imaginary output */
 } fk_ew_args;
 
 /* registry, defined by the generated instantiation files */

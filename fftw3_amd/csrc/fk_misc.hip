@@ -175,6 +175,18 @@ __global__ void k_ew(fk_ew_args a)
     case FK_EW_COPY_R:
         ro[j * a.os] = ri[j * a.is];
         break;
+    case FK_EW_SPLIT_GATHER: {
+        const R *ri2 = reinterpret_cast<const R *>(a.in2) + bi;
+        co[j * a.os] = mk(ri[j * a.is], ri2[j * a.is]);
+        break;
+    }
+    case FK_EW_SPLIT_SCATTER: {
+        R *ro2 = reinterpret_cast<R *>(a.out2) + bo;
+        const cx<R> v = ci[j * a.is];
+        ro[j * a.os] = v.x;
+        ro2[j * a.os] = v.y;
+        break;
+    }
     case FK_EW_BS_PREP: {
         cx<R> v = mk((R)0, (R)0);
         if (j < a.p0) {

@@ -289,10 +289,11 @@ static int launch_round(fk_server *s, sround *r)
     for (i = 0; i < r->nreq; ++i) {
         sreq *q = r->reqs[i];
         fk_plan *P = q->plan;
-        void *buf[FK_BUF_S0 + FK_MAX_SCRATCH];
+        void *buf[FK_NBUF];
         if (!in_menu(s, P)) continue;
         buf[FK_BUF_IN] = (void *)q->d.in;
         buf[FK_BUF_OUT] = q->out;
+        buf[FK_BUF_IN2] = buf[FK_BUF_OUT2] = 0;
         for (j = 0; j < FK_MAX_SCRATCH; ++j) buf[FK_BUF_S0 + j] = q->scr[j];
         for (j = 0; j < P->npass; ++j) {
             const fk_pass *ps = P->pass + j;

@@ -21,8 +21,13 @@
  *   print_plan :429-436 malloc/alloc/free :438-447
  *   flops/estimate_cost/cost :449-456 alignment_of :458 version :461
  *
+ * Split-complex guru planners (plan_guru{,64}_split_dft{,_r2c,_c2r}) and
+ * execute_split_dft{,_r2c,_c2r} follow api/plan-guru-split-dft*.c and
+ * api/execute-split-dft*.c: strides count reals, no sign argument (swap the
+ * real and imaginary pointers of each pair for the backward transform).
+ *
  * Not provided by this engine (planners return NULL): r2r kinds
- * (plan_*r2r*), split-array guru planners (plan_guru*_split_*).
+ * (plan_*r2r*).
  */
 #ifndef FFTW3_H
 #define FFTW3_H
