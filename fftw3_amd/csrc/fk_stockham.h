@@ -284,11 +284,29 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
         R *outr = reinterpret_cast<R *>(a.out) + bout;
         const cx<R> *wk = reinterpret_cast<const cx<R> *>(a.aux);
-        for (int k = u; k <= L; k += TPL) {
-            cx<R> v = act ? in[k * a.is] : mk((R)0, (R)0);
-            if (k == 0 || k == L) v.y = 0;
-            lds_at(k) = v;
-        }
+        /* loads in groups of 8 (compile-time count) so that each group is
+           in flight at once, then its LDS stores */
+        constexpr int NK = (L + 1 + TPL - 1) / TPL;
+        constexpr int GK = 8;
+        sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
+            cx<R> xin[cnt];
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if (k <= L) xin[i] = act ? in[(long long)k * a.is] : mk((R)0, (R)0);
+            });
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if (k <= L) {
+                    cx<R> v = xin[i];
+                    if (k == 0 || k == L) v.y = 0;
+                    lds_at(k) = v;
+                }
+            });
+        });
         __syncthreads();
         auto pre = [&](int j) -> cx<R> {
             const cx<R> xk = lds_at(j), xc = lds_at(L - j);
