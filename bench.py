@@ -491,6 +491,12 @@ def main():
     for k, (t, by, c) in sorted(groups.items(), key=lambda kv: -kv[1][0]):
         log("  %-90s %8.3f ms %7.1f GB/s x%d" % (k[:90], t, by / (t * 1e-3) / 1e9, c))
 
+    # ---- single-size batched c2c (1-D FP64 N=4096 x 1024, out of place),
+    # the configuration of the >= 70 % roofline target; not part of the step
+    c1 = single_size_c2c(fa, L, oracle)
+    log("single-size c2c 4096 x 1024 fp64: %.4f ms  %.1f GB/s  frac %.3f  (%s)"
+        % (c1["ms"], c1["GB_s"], c1["frac"], c1["passes"][0]))
+
     # ---- CPU baseline: oracle double path on the host (bounded sample) ----
     cpu = None
     if not args.no_cpu:
@@ -517,6 +523,7 @@ def main():
             "bytes_per_step": bytes_step,
             "roundtrip_rel_l2": {"c2c3d_fp64": rt3, "r2c_c2r_2d_fp32": rt2},
             "sub_ms": {k: round(v, 4) for k, v in sub.items()},
+            "single_size_c2c_fp64_4096x1024": c1,
         },
         "roofline": {
             "bound": "hbm",
@@ -534,6 +541,42 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)
+
+
+def single_size_c2c(fa, L, oracle, n=4096, howmany=1024, iters=50):
+    """1-D c2c FP64, N=4096, howmany=1024, contiguous, out of place, seed 1
+    (FFTW_MEASURE): event-timed mean of `iters` back-to-back executes,
+    algorithmic bytes 2*N*howmany*16, and a parity check of the first
+    transform against the oracle."""
+    import numpy as np
+    tot = n * howmany
+    a = fa.DeviceBuffer(tot * 16)
+    b = fa.DeviceBuffer(tot * 16)
+    a.fill_uniform("double", 2 * tot, 1, 0)
+    p = fa.plan_many_dft("double", [n], howmany, a.ptr, None, 1, n, b.ptr, None, 1, n,
+                         fa.FFTW_FORWARD, fa.FFTW_MEASURE)
+    a.fill_uniform("double", 2 * tot, 1, 0)
+    p.execute()
+    L.fftw_amd_synchronize()
+    x = oracle.uniform(2 * n, 1, 0).view(np.complex128)
+    err = oracle.rel_l2(b.download(np.complex128, n), oracle.fftn(x, -1))
+    e0, e1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
+    for _ in range(3):
+        p.execute()
+    L.fftw_amd_synchronize()
+    L.fftw_amd_event_record(e0, None)
+    for _ in range(iters):
+        p.execute()
+    L.fftw_amd_event_record(e1, None)
+    L.fftw_amd_synchronize()
+    ms = L.fftw_amd_event_elapsed(e0, e1) / iters
+    L.fftw_amd_event_destroy(e0)
+    L.fftw_amd_event_destroy(e1)
+    gbs = 2.0 * tot * 16 / (ms * 1e-3) / 1e9
+    rec = {"ms": round(ms, 5), "GB_s": round(gbs, 1), "frac": round(gbs / PEAK_HBM, 4),
+           "rel_l2_first": err, "passes": p.pass_info()}
+    p.destroy()
+    return rec
 
 
 def kernel_symbol(info):

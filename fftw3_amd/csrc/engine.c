@@ -203,7 +203,13 @@ static long long isqrt_ceil(long long n)
 /* FP64 two-level table of W_N^e: lo (b entries) then hi (N/b+1 entries) */
 static const double *tw2_table(long long N, long long *b_out, hipStream_t st)
 {
+    /* split b ~ sqrt(N), a power of two when N is one (the kernels then
+       index the two levels by mask and shift) */
     long long b = isqrt_ceil(N);
+    if ((N & (N - 1)) == 0) {
+        b = 1;
+        while (b * b < N) b <<= 1;
+    }
     tbl *t = tbl_find(T_TW2, 0, N, b, 0, 0);
     *b_out = b;
     if (t) return (const double *)t->dev;
@@ -553,20 +559,6 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb,
                        const fk_dim *bin, int ib, long long io, int ob, long long oo, int swi,
                        int swo, int slot);
 
-/* tw: FK_TW_OUT setup helper */
-static int set_tw_out(ctx *c, fk_pass *proto, long long N, long long twm)
-{
-    long long b;
-    const double *t = tw2_table(N, &b, c->st);
-    if (!t) return -1;
-    proto->a.tlo = t;
-    proto->a.thi = t + 2 * b;
-    proto->a.tN = N;
-    proto->a.tb = b;
-    proto->a.twm = twm;
-    return 0;
-}
-
 static int plan_bluestein(ctx *c, long long n, long long is, long long os, int nb,
                           const fk_dim *b, int ib, long long io, int ob, long long oo, int swi,
                           int swo, int slot)
@@ -848,7 +840,6 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
                 p->a.thi = t + 2 * tb;
                 p->a.tN = n;
                 p->a.tb = tb;
-                p->a.twm = 1;
             }
         }
         /* pass B: n2-point transforms of rows k1, written to k1 + n1*k2 */

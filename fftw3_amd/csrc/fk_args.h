@@ -27,7 +27,7 @@ enum {
 #define FK_SWAP_IN   1u   /* swap re/im on load  (backward transform) */
 #define FK_SWAP_OUT  2u   /* swap re/im on store (backward transform) */
 #define FK_TW_OUT    4u   /* multiply output bin k of line with batch index
-                             i0 by W_N^(k*i0*twm) (four-step twiddle) */
+                             i0 by W_N^(k*i0) (four-step twiddle) */
 #define FK_SCALE     8u   /* multiply output by scale */
 #define FK_REAL_UNIT 16u  /* R2C: real input stride is 1 / C2R: output stride 1 */
 
@@ -39,10 +39,10 @@ typedef struct {
     long long nb[3];          /* batch counts */
     long long nlines;         /* nb[0]*nb[1]*nb[2] */
     const void *tw;           /* per-stage twiddles, working precision */
-    /* four-step output twiddle W_N^e, e = (k*i0*twm) mod N, from an FP64
+    /* four-step output twiddle W_N^e, e = k*i0 < N, from an FP64
        two-level table: W_N^e = tlo[e % tb] * thi[e / tb] */
     const double *tlo, *thi;
-    long long tN, tb, twm;
+    long long tN, tb;
     /* FK_IO_R2C / FK_IO_C2R: W_n^k, k <= L (working precision)
        FK_IO_BLUE: chirp c_m (m < nvalid), then FFT_L(conj chirp)/L */
     const void *aux, *aux2;

@@ -256,10 +256,27 @@ __device__ __forceinline__ void cexp_fp64(long long m, long long n, double &c,
 template <typename R>
 __device__ __forceinline__ cx<R> tw2(const fk_args &a, long long e)
 {
-    long long N = a.tN;
-    e %= N;
-    if (e < 0) e += N;
-    const double *lo = a.tlo + 2 * (e % a.tb), *hi = a.thi + 2 * (e / a.tb);
+    /* W_N^e from the two-level FP64 table, W_N^e = lo[e % b] * hi[e / b].
+       Four-step exponents are e = k*i0 < N; the reduction below only runs
+       for exponents outside [0, N). */
+    const long long N = a.tN, b = a.tb;
+    if (e < 0 || e >= N) {
+        e %= N;
+        if (e < 0) e += N;
+    }
+    long long il, ih;
+    if ((b & (b - 1)) == 0) {                  /* power-of-two split */
+        il = e & (b - 1);
+        ih = e >> __builtin_ctzll((unsigned long long)b);
+    } else if (N <= 0x7fffffffLL) {            /* 32-bit division */
+        const unsigned ue = (unsigned)e, ub = (unsigned)b, q = ue / ub;
+        ih = q;
+        il = ue - q * ub;
+    } else {
+        ih = e / b;
+        il = e - ih * b;
+    }
+    const double *lo = a.tlo + 2 * il, *hi = a.thi + 2 * ih;
     double c = lo[0] * hi[0] - lo[1] * hi[1];
     double s = lo[0] * hi[1] + lo[1] * hi[0];
     return mk((R)c, (R)s);

@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(64 * ((S == 3) ? 8 : 1)) k_mfma(fk_args a)
                 if (st < S - 1) {
                     lds[idx(pos)] = y;
                 } else if (act) {
-                    if (a.flags & FK_TW_OUT) y = mul(y, tw2<float>(a, (long long)pos * i0 * a.twm));
+                    if (a.flags & FK_TW_OUT) y = mul(y, tw2<float>(a, (long long)pos * i0));
                     if (a.flags & FK_SCALE) { y.x *= (float)a.scale; y.y *= (float)a.scale; }
                     out[bout + pos * a.os] = swo ? swp(y) : y;
                 }

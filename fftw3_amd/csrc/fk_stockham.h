@@ -217,7 +217,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
 
     /* output post-op: twiddle, scale, swap */
     auto post = [&](int k, cx<R> v) -> cx<R> {
-        if (fl & FK_TW_OUT) v = mul(v, tw2<R>(a, (long long)k * i0 * a.twm));
+        if (fl & FK_TW_OUT) v = mul(v, tw2<R>(a, (long long)k * i0));
         if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
         return swo ? swp(v) : v;
     };
