@@ -589,9 +589,6 @@ def kernel_symbol(info):
     rads = kv["rad"].split("x")
     real = "double" if f[1] == "f64" else "float"
     mode = 0 if f[4] == "row" else 1
-    if f[5] == "loop":
-        return "void fk::stockham_loop<%s, %s, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
-            real, kv["nl"], mode, rads[0], ", ".join(rads), kv["tpl"])
     io = {"plain": 0, "tout": 1, "r2c": 2, "c2r": 3, "bluestein": 4}[f[5]]
     return "void fk::stockham<%s, %s, %d, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
         real, kv["nl"], mode, io, rads[0], ", ".join(rads), kv["tpl"])

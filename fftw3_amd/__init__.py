@@ -28,6 +28,7 @@ FFTW_PATIENT = 1 << 5
 FFTW_ESTIMATE = 1 << 6
 FFTW_WISDOM_ONLY = 1 << 21
 FFTW_AMD_MFMA = 1 << 24
+FFTW_AMD_BLOCKED = 1 << 25
 
 _lib = None
 

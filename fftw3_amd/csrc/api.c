@@ -242,10 +242,15 @@ PLAN X(plan_guru64_split_dft)(int rank, const X(iodim64) *dims, int howmany_rank
 
 static fk_plan *P_(PLAN p) { return (fk_plan *)p; }
 
+/* api/execute.c:23-27: the planning arrays (both halves of split ones) */
 void X(execute)(const PLAN p)
 {
-    if (!p) return;
-    fk_execute_or_die(P_(p), P_(p)->in, P_(p)->out);
+    fk_plan *P = P_(p);
+    if (!P) return;
+    if (fk_execute4(P, P->in, P->in2, P->out, P->out2)) {
+        fprintf(stderr, "fftw3_amd: %s\n", fk_last_error());
+        abort();
+    }
 }
 
 /* new-array execute: api/execute-dft.c:25-32 (sign is part of the plan) */

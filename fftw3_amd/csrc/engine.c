@@ -38,8 +38,6 @@ static hipStream_t g_stream = 0;
 static int g_async = 0;
 static double g_timelimit = -1.0;
 static __thread char g_err[512];
-static int g_ncu = 256;        /* compute units of device 0 */
-static int *g_occ = 0;         /* resident workgroups per CU of each looped kernel */
 
 static void set_err(const char *fmt, ...)
 {
@@ -74,39 +72,8 @@ int fk_init(void)
             }
         }
     }
-    if (hipDeviceGetAttribute(&g_ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
-        g_ncu < 1)
-        g_ncu = 256;
-    g_occ = (int *)calloc((size_t)g_nreg, sizeof(int));
-    if (!g_occ) {
-        set_err("out of host memory");
-        return -1;
-    }
-    for (i = 0; i < g_nreg; ++i) {
-        int n = 0;
-        if (!g_reg[i].loop) continue;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, g_reg[i].fn, g_reg[i].tpl * g_reg[i].nl,
-                                                         g_reg[i].lds) != hipSuccess) {
-            (void)hipGetLastError();
-            n = 1;
-        }
-        g_occ[i] = n < 1 ? 1 : n;
-    }
     g_inited = 1;
     return 0;
-}
-
-/* Grid of a looped (persistent) kernel: the workgroups resident at once,
-   rounded to a multiple of 8 (one share per XCD), no more than the tiles
-   need. */
-static unsigned loop_grid(const fk_kernel *k, long long nlines)
-{
-    long long ntiles = (nlines + k->nl - 1) / k->nl;
-    long long occ = (g_occ && k >= g_reg && k < g_reg + g_nreg) ? g_occ[k - g_reg] : 1;
-    long long g = (occ < 1 ? 1 : occ) * g_ncu, need = (ntiles + 7) / 8 * 8;
-    if (g > need) g = need;
-    g = g / 8 * 8;
-    return (unsigned)(g < 8 ? 8 : g);
 }
 
 /* ------------------------------------------------------------------ */
@@ -328,7 +295,7 @@ static int smooth_plannable(int prec, long long n, int depth)
 /* Planner decision kinds, ordered by the patience level that explores them
    (MEASURE explores kernel geometries and thread mappings, PATIENT also
    alternative two-pass splits). */
-enum { DEC_VARIANT = 1, DEC_MODE = 1, DEC_SPLIT = 2 };
+enum { DEC_VARIANT = 1, DEC_MODE = 1, DEC_CHUNK = 1, DEC_SPLIT = 2 };
 #define FK_MAX_DECIDE 32
 
 typedef struct {
@@ -502,8 +469,7 @@ static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b, si
         p->o2off = o2off;
         if (p->type == FK_PASS_STOCKHAM) {
             fill_batch3(p->a.nb, p->a.ids, p->a.ods, &p->a.nlines, nb < 3 ? nb : 3, b);
-            p->grid = p->k->loop ? loop_grid(p->k, p->a.nlines)
-                                 : (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
+            p->grid = (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
             if (p->a.nlines == 0) --c->P->npass;
         } else {
             fill_batch3(p->e.nb, p->e.ids, p->e.ods, &p->e.nlines, nb < 3 ? nb : 3, b);
@@ -900,6 +866,188 @@ static int plan_c2c_dims(ctx *c, int rank, const fk_dim *dims, int vrank, const 
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* cache blocking of multi-dimensional plans                           */
+/*
+ * MI355X's 256 MB memory-side cache (Infinity Cache) keeps a working set of
+ * that order between two passes.  A multi-dimensional plan can therefore
+ * run its passes chunk by chunk, so that the intermediate of a chunk is
+ * re-read from that cache instead of HBM:
+ *   - batched problems (rank >= 2, howmany > 1): every pass, per chunk of
+ *     transforms;
+ *   - unbatched rank >= 3 problems: the passes of dims 1..rank-1 per slab
+ *     of dim-0 planes, then the dim-0 pass over the whole array.
+ * The chunk size is a planner decision timed under FFTW_MEASURE; the
+ * heuristic (FFTW_ESTIMATE) plans the whole problem at once.
+ */
+static const long long chunk_targets[] = {64LL << 20, 128LL << 20};
+
+/* Offsets the arrays' byte offsets of passes [from, npass) (the chunk's
+   position in the caller's arrays). */
+static void shift_passes(fk_plan *P, int from, long long in_off, long long out_off)
+{
+    int i;
+    for (i = from; i < P->npass; ++i) {
+        fk_pass *p = P->pass + i;
+        if (p->ibuf == FK_BUF_IN) p->ioff += in_off;
+        else if (p->ibuf == FK_BUF_OUT) p->ioff += out_off;
+        if (p->obuf == FK_BUF_IN) p->ooff += in_off;
+        else if (p->obuf == FK_BUF_OUT) p->ooff += out_off;
+    }
+}
+
+/* Units (transforms or planes) per chunk: all of them, or as many as fit
+   a cache target; a decision when there is a choice. */
+static long long chunk_units(ctx *c, long long nunits, double unit_bytes)
+{
+    long long cand[4];
+    int nc = 0, t;
+    cand[nc++] = nunits;
+    for (t = 0; t < (int)(sizeof(chunk_targets) / sizeof(*chunk_targets)); ++t) {
+        long long k = (long long)((double)chunk_targets[t] / unit_bytes);
+        if (k < 1) k = 1;
+        if (k < nunits && k != cand[nc - 1]) cand[nc++] = k;
+    }
+    if (nc == 1) return cand[0];
+    if ((c->P->flags & FK_BLOCKED) && !(c->choice && c->ci < c->nchoice)) {
+        /* FFTW_AMD_BLOCKED: the first cache-sized chunk is the default */
+        int i = c->ci++;
+        if (i < FK_MAX_DECIDE) {
+            c->nalt[i] = nc;
+            c->dkind[i] = DEC_CHUNK;
+            c->P->choice[i] = 1;
+        }
+        return cand[1];
+    }
+    return cand[decide(c, nc, DEC_CHUNK)];
+}
+
+typedef int (*chunk_fn)(ctx *c, const fk_problem *q, int sw);
+
+/* Plans q in chunks of `k` indices of vdim v (balanced chunks); every
+   chunk replays the decisions of the first. */
+static int plan_vchunks(ctx *c, const fk_problem *pb, int v, long long k, chunk_fn f, int sw,
+                        size_t isz, size_t osz)
+{
+    long long n = pb->vdims[v].n, nch = (n + k - 1) / k, h0 = 0, j;
+    int ci0 = c->ci, ci1 = c->ci;
+    for (j = 0; j < nch; ++j) {
+        fk_problem q = *pb;
+        long long cnt = n / nch + (j < n % nch);
+        int before = c->P->npass;
+        q.vdims[v].n = cnt;
+        c->ci = ci0;
+        if (f(c, &q, sw)) return -1;
+        if (c->ci > ci1) ci1 = c->ci;
+        shift_passes(c->P, before, h0 * pb->vdims[v].is * (long long)isz,
+                     h0 * pb->vdims[v].os * (long long)osz);
+        h0 += cnt;
+    }
+    c->ci = ci1;
+    return 0;
+}
+
+static int chunk_c2c(ctx *c, const fk_problem *q, int sw)
+{
+    return plan_c2c_dims(c, q->rank, q->dims, q->vrank, q->vdims, FK_BUF_IN, FK_BUF_OUT, sw);
+}
+
+static int plan_r2c(ctx *c, const fk_problem *pb);
+static int plan_c2r(ctx *c, const fk_problem *pb);
+static int chunk_r2c(ctx *c, const fk_problem *q, int sw) { (void)sw; return plan_r2c(c, q); }
+static int chunk_c2r(ctx *c, const fk_problem *q, int sw) { (void)sw; return plan_c2r(c, q); }
+
+/* vdim to chunk over: the one with the most indices */
+static int chunk_vdim(const fk_problem *pb)
+{
+    int i, v = -1;
+    for (i = 0; i < pb->vrank; ++i)
+        if (pb->vdims[i].n > 1 && (v < 0 || pb->vdims[i].n > pb->vdims[v].n)) v = i;
+    return v;
+}
+
+/* Cache-blocked c2c (rank >= 2); returns 1 if it planned the problem, 0 if
+   the whole-problem plan is the choice, -1 on error. */
+static int plan_c2c_blocked(ctx *c, const fk_problem *pb, int sw)
+{
+    fk_plan *P = c->P;
+    size_t cs = csz(P->prec);
+    long long N = 1, H = 1, k;
+    int i, v;
+    for (i = 0; i < pb->rank; ++i) N *= pb->dims[i].n;
+    for (i = 0; i < pb->vrank; ++i) H *= pb->vdims[i].n;
+    if (pb->rank < 2) return 0;
+    v = chunk_vdim(pb);
+    if (v >= 0) {
+        k = chunk_units(c, pb->vdims[v].n, (double)N * cs * (double)(H / pb->vdims[v].n));
+        if (k >= pb->vdims[v].n) return 0;
+        desc_add(P, "(blocked %lld of %lld)", k, pb->vdims[v].n);
+        return plan_vchunks(c, pb, v, k, chunk_c2c, sw, cs, cs) ? -1 : 1;
+    }
+    if (pb->rank < 3) return 0;
+    {
+        long long n0 = pb->dims[0].n, nch, h0 = 0, j;
+        int ci0, ci1, nb = 0;
+        fk_dim b[FK_MAXRANK * 2];
+        k = chunk_units(c, n0, (double)(N / n0) * cs);
+        if (k >= n0) return 0;
+        desc_add(P, "(blocked slabs %lld of %lld)", k, n0);
+        nch = (n0 + k - 1) / k;
+        ci0 = ci1 = c->ci;
+        for (j = 0; j < nch; ++j) {
+            fk_dim vd[FK_MAXRANK + 1];
+            long long cnt = n0 / nch + (j < n0 % nch);
+            int before = P->npass;
+            memcpy(vd, pb->vdims, sizeof(fk_dim) * pb->vrank);
+            vd[pb->vrank].n = cnt;
+            vd[pb->vrank].is = pb->dims[0].is;
+            vd[pb->vrank].os = pb->dims[0].os;
+            c->ci = ci0;
+            if (plan_c2c_dims(c, pb->rank - 1, pb->dims + 1, pb->vrank + 1, vd, FK_BUF_IN,
+                              FK_BUF_OUT, sw))
+                return -1;
+            if (c->ci > ci1) ci1 = c->ci;
+            shift_passes(P, before, h0 * pb->dims[0].is * (long long)cs,
+                         h0 * pb->dims[0].os * (long long)cs);
+            h0 += cnt;
+        }
+        c->ci = ci1;
+        /* dim 0 over the whole array, in place on the output */
+        for (i = 1; i < pb->rank; ++i) {
+            b[nb].n = pb->dims[i].n;
+            b[nb].is = b[nb].os = pb->dims[i].os;
+            ++nb;
+        }
+        for (i = 0; i < pb->vrank; ++i) {
+            b[nb].n = pb->vdims[i].n;
+            b[nb].is = b[nb].os = pb->vdims[i].os;
+            ++nb;
+        }
+        if (plan_c2c_1d(c, n0, pb->dims[0].os, pb->dims[0].os, nb, b, FK_BUF_OUT, 0, FK_BUF_OUT,
+                        0, sw, sw, 0))
+            return -1;
+        return 1;
+    }
+}
+
+/* Cache-blocked r2c / c2r (rank >= 2, batched): 1 planned, 0 not, -1 error */
+static int plan_real_blocked(ctx *c, const fk_problem *pb)
+{
+    fk_plan *P = c->P;
+    size_t cs = csz(P->prec), rs = rsz(P->prec);
+    long long Nh = 1, H = 1, k;
+    int i, v = chunk_vdim(pb), r = pb->rank;
+    if (r < 2 || v < 0) return 0;
+    for (i = 0; i < r - 1; ++i) Nh *= pb->dims[i].n;
+    Nh *= pb->dims[r - 1].n / 2 + 1;
+    for (i = 0; i < pb->vrank; ++i) H *= pb->vdims[i].n;
+    k = chunk_units(c, pb->vdims[v].n, (double)Nh * cs * (double)(H / pb->vdims[v].n));
+    if (k >= pb->vdims[v].n) return 0;
+    desc_add(P, "(blocked %lld of %lld)", k, pb->vdims[v].n);
+    if (pb->kind == FK_R2C) return plan_vchunks(c, pb, v, k, chunk_r2c, 0, rs, cs) ? -1 : 1;
+    return plan_vchunks(c, pb, v, k, chunk_c2r, 0, cs, rs) ? -1 : 1;
+}
+
 static int plan_copy(ctx *c, int vrank, const fk_dim *vdims, int op, size_t esz)
 {
     fk_pass proto;
@@ -1279,7 +1427,7 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
     fk_problem pb = *pb0;
     fk_plan *P;
     ctx c;
-    int i, ok = 0;
+    int i, ok = 0, rb;
     if (fk_init()) return 0;
     if (!dims_kosher(&pb)) {
         set_err("invalid dimensions");
@@ -1332,8 +1480,10 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
             desc_add(P, "(dft-%s rank=%d", sw ? "bwd" : "fwd", pb.rank);
             for (i = 0; i < pb.rank; ++i) desc_add(P, " %lld", pb.dims[i].n);
             desc_add(P, ")");
-            if (plan_c2c_dims(&c, pb.rank, pb.dims, pb.vrank, pb.vdims, FK_BUF_IN, FK_BUF_OUT,
-                              sw))
+            rb = plan_c2c_blocked(&c, &pb, sw);
+            if (rb < 0) goto fail;
+            if (!rb && plan_c2c_dims(&c, pb.rank, pb.dims, pb.vrank, pb.vdims, FK_BUF_IN,
+                                     FK_BUF_OUT, sw))
                 goto fail;
         }
     } else if (pb.kind == FK_R2C) {
@@ -1352,7 +1502,7 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
             proto.ibuf = FK_BUF_IN;
             proto.obuf = FK_BUF_OUT;
             emit_loops(&c, &proto, nb, b, rsz(pb.prec), csz(pb.prec));
-        } else if (plan_r2c(&c, &pb)) {
+        } else if ((rb = plan_real_blocked(&c, &pb)) < 0 || (!rb && plan_r2c(&c, &pb))) {
             goto fail;
         }
     } else {
@@ -1375,7 +1525,7 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
             proto.ibuf = FK_BUF_IN;
             proto.obuf = FK_BUF_OUT;
             emit_loops(&c, &proto, nb, b, csz(pb.prec), rsz(pb.prec));
-        } else if (plan_c2r(&c, &pb)) {
+        } else if ((rb = plan_real_blocked(&c, &pb)) < 0 || (!rb && plan_c2r(&c, &pb))) {
             goto fail;
         }
     }
@@ -1415,7 +1565,7 @@ static unsigned long long problem_hash(const fk_problem *pb)
     v[m++] = pb->kind;
     v[m++] = pb->prec;
     v[m++] = pb->kind == FK_C2C ? pb->sign : 0;
-    v[m++] = (pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT | FK_MFMA)) |
+    v[m++] = (pb->flags & (FK_PRESERVE_INPUT | FK_DESTROY_INPUT | FK_MFMA | FK_BLOCKED)) |
              ((long long)pb->split << 40);
     v[m++] = pb->in == pb->out;
     v[m++] = (long long)((((uintptr_t)pb->in) | ((uintptr_t)pb->out)) % 16);
@@ -1763,8 +1913,7 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
                         "stockham %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
                         "is=%lld os=%lld flags=%u",
                         ps->k->prec ? "f32" : "f64", ps->k->L, rad,
-                        ps->k->mode == FK_ROW ? "row" : "col",
-                        ps->k->loop ? "loop" : io_name(ps->k->io),
+                        ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
                         ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
                         ps->a.os, ps->a.flags);
     }

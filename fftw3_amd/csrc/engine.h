@@ -50,6 +50,7 @@ enum { FK_PASS_STOCKHAM = 0, FK_PASS_EW = 1 };
 #define FK_ESTIMATE (1U << 6)
 #define FK_WISDOM_ONLY (1U << 21)
 #define FK_MFMA (1U << 24)          /* project flag: split-precision radix-16 */
+#define FK_BLOCKED (1U << 25)       /* project flag: cache-blocked multi-D plans */
 
 typedef struct { long long n, is, os; } fk_dim;
 

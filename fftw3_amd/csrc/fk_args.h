@@ -65,8 +65,6 @@ typedef struct {
     int twsz;          /* complex entries in the per-stage twiddle table */
     const void *fn;    /* kernel host stub for hipLaunchKernel */
     int variant;       /* 0 = default geometry; >0 autotuner alternatives */
-    int loop;          /* 1: persistent software-pipelined form (grid = resident
-                          workgroups, a multiple of 8, each looping over tiles) */
 } fk_kernel;
 
 /* elementwise passes around the Stockham kernels (multi-pass paths) */

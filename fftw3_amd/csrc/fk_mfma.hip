@@ -209,7 +209,6 @@ int fkm_mfma_kernel(int L, fk_kernel *k)
     k->fn = S == 1 ? (const void *)&k_mfma<1> : S == 2 ? (const void *)&k_mfma<2>
                                                      : (const void *)&k_mfma<3>;
     k->variant = 100;
-    k->loop = 0;
     return 0;
 }
 

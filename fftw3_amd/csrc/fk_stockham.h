@@ -71,20 +71,10 @@ struct Geo {
     }
 };
 
-/* Workgroup barriers.  SyncAll is __syncthreads (it also waits for every
-   outstanding global access of the thread); SyncLds orders LDS only, so
-   global loads issued before it stay in flight across the barrier.  Every
-   barrier of the Stockham stages guards LDS data only. */
+/* Workgroup barrier of the Stockham stages (a policy parameter so that a
+   kernel may substitute its own). */
 struct SyncAll {
     __device__ __forceinline__ static void sync() { __syncthreads(); }
-};
-struct SyncLds {
-    __device__ __forceinline__ static void sync()
-    {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        __builtin_amdgcn_s_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    }
 };
 
 /*
@@ -372,234 +362,6 @@ constexpr unsigned stockham_lds()
     using G = Geo<R, RL::L(), TPL, NL, MODE, PAD>;
     return G::lds_elems() * sizeof(cx<R>) + ((NL + 1) & ~1) * sizeof(long long) +
            (NT > TPL * NL ? 64 * sizeof(cx<R>) : 0);
-}
-
-/* ---- raw buffer access: 64-bit base in SGPRs (a buffer resource), a
-   32-bit per-lane byte offset and a 32-bit uniform byte offset ---- */
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p)
-{
-    /* stride 0, 2^32-1 bytes, gfx9 raw-buffer format word */
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)0xFFFFFFFFu,
-                                             0x00020000);
-}
-template <typename R>
-__device__ __forceinline__ cx<R> buf_ld(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff)
-{
-    if constexpr (sizeof(R) == 8) {
-        auto w = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)soff, 0);
-        return __builtin_bit_cast(cx<R>, w);
-    } else {
-        auto w = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 0);
-        return __builtin_bit_cast(cx<R>, w);
-    }
-}
-template <typename R>
-__device__ __forceinline__ void buf_st(cx<R> v, __amdgpu_buffer_rsrc_t rs, unsigned voff,
-                                       unsigned soff)
-{
-    if constexpr (sizeof(R) == 8) {
-        typedef unsigned u4 __attribute__((ext_vector_type(4)));
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), rs, (int)voff, (int)soff, 0);
-    } else {
-        typedef unsigned u2 __attribute__((ext_vector_type(2)));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, (int)voff, (int)soff, 0);
-    }
-}
-
-/* waves per SIMD of a looped kernel when as many workgroups are resident
-   per CU as LDS and threads allow: its launch bounds ask the compiler for
-   registers that permit that many (second argument of HIP's
-   __launch_bounds__ = minimum waves per execution unit) */
-template <typename R, int NL, int MODE, int PAD, class RL, int TPL>
-constexpr int stockham_loop_waves()
-{
-    using G = Geo<R, RL::L(), TPL, NL, MODE, PAD>;
-    constexpr unsigned lds = (G::lds_elems() + RL::twsize()) * sizeof(cx<R>);
-    constexpr int by_lds = (int)((160u * 1024u) / lds);
-    constexpr int by_thr = 2048 / (TPL * NL);
-    constexpr int m = by_lds < by_thr ? by_lds : by_thr;
-    constexpr int w = (m < 1 ? 1 : m) * ((TPL * NL + 63) / 64) / 4;
-    return w < 1 ? 1 : w;
-}
-
-/* Last Stockham stage (LDS in, HBM out): butterfly b < Ns of radix Rs
-   writes outputs b + r*Ns, handed to st(b, r, value) with r a constant. */
-template <typename R, class RL, int L, int TPL, class LoadF, class StoreF>
-__device__ __forceinline__ void stage_last(int u, const cx<R> *__restrict__ tw, LoadF &&ld,
-                                           StoreF &&st)
-{
-    constexpr int S = RL::n - 1;
-    constexpr int Rs = RL::r[S];
-    constexpr int Ns = RL::ns(S);
-    constexpr int NB = L / Rs;
-    constexpr int BPT = (NB + TPL - 1) / TPL;
-    static_assert(NB == Ns, "last stage: one butterfly per output group");
-    cx<R> v[BPT][Rs];
-    sfor<0, BPT>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        const int b = u + i * TPL;
-        if ((NB % TPL == 0) || b < NB) {
-            sfor<0, Rs>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[i][r] = ld(b + r * NB);
-            });
-        }
-    });
-    sfor<0, BPT>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        const int b = u + i * TPL;
-        if ((NB % TPL == 0) || b < NB) {
-            const cx<R> *w = tw + RL::twoff(S) + b * (Rs - 1);
-            sfor<1, Rs>([&](auto rc) {
-                constexpr int r = decltype(rc)::value;
-                v[i][r] = mul(v[i][r], w[r - 1]);
-            });
-            Dft<R, Rs, 1>::run(v[i]);
-            sfor<0, Rs>([&](auto rc) { st(b, rc, v[i][decltype(rc)::value]); });
-        }
-    });
-}
-
-/*
- * Software-pipelined persistent form of the PLAIN pass (ROW or COL
- * mapping; no four-step output twiddle).  The grid holds about as many
- * workgroups as are resident at once, a multiple of 8; XCD x (the
- * workgroups b with b % 8 == x, the hardware's round-robin placement) owns
- * a contiguous eighth of the tiles and its workgroups take them in turn.
- * While a workgroup runs the LDS stages and the stores of one tile, the
- * stage-0 inputs of its next tile are already loading into the other of
- * two register sets: the stage twiddles are copied to LDS once, and the
- * barriers order LDS only, so nothing waits on the prefetch before the
- * iteration that consumes it.  HBM is addressed through buffer resources
- * (per-lane 32-bit byte offsets, the radix term of every stage-0 load and
- * last-stage store as a uniform offset), so the host uses this form only
- * when every stride is non-negative, every byte offset of the pass stays
- * below 2^32 and nlines < 2^31 (line indices are decomposed in 32 bits).
- */
-template <typename R, int NL, int MODE, int PAD, class RL, int TPL>
-__global__ void __launch_bounds__(TPL * NL, (stockham_loop_waves<R, NL, MODE, PAD, RL, TPL>()))
-stockham_loop(fk_args a)
-{
-    extern __shared__ unsigned char smem_raw[];
-    constexpr int L = RL::L();
-    constexpr int NR = TPL * NL;
-    constexpr int R0 = RL::r[0];
-    constexpr int NB0 = L / R0;
-    constexpr int BPT0 = (NB0 + TPL - 1) / TPL;
-    constexpr unsigned CS = sizeof(cx<R>);
-    constexpr int NS_LAST = RL::ns(RL::n - 1);
-    static_assert(RL::n >= 2, "looped kernels have at least two stages");
-    using G = Geo<R, L, TPL, NL, MODE, PAD>;
-    cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
-    cx<R> *twl = lds + G::lds_elems();
-
-    const int tid = threadIdx.x;
-    const int lt = (MODE == FK_ROW) ? tid / TPL : tid % NL;
-    const int u = (MODE == FK_ROW) ? tid % TPL : tid / NL;
-    const unsigned fl = a.flags;
-    const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
-    const auto rin = buf_rsrc(a.in);
-    const auto rout = buf_rsrc(a.out);
-    const unsigned isb = (unsigned)a.is * CS, osb = (unsigned)a.os * CS;
-
-    {
-        const cx<R> *twg = reinterpret_cast<const cx<R> *>(a.tw);
-        for (int i = tid; i < RL::twsize(); i += NR) twl[i] = twg[i];
-    }
-
-    const long long ntiles = (a.nlines + NL - 1) / NL;
-    const long long g8 = gridDim.x / 8;
-    const long long x = blockIdx.x % 8;
-    const long long hi = ntiles * (x + 1) / 8;
-    long long tile = ntiles * x / 8 + blockIdx.x / 8;
-
-    /* stage-0 inputs of tile t into v: plain loads that nothing reads
-       before the iteration that consumes them (the re/im swap is applied
-       there); lines past the end (last tile) read line 0, which always
-       exists, and store nothing.  Returns the byte offset of the output
-       line, 0xFFFFFFFF if inactive. */
-    const unsigned n0 = (unsigned)a.nb[0], n1 = (unsigned)a.nb[1];
-    auto load = [&](cx<R> (&v)[BPT0][R0], long long t) -> unsigned {
-        const long long line = t * NL + lt;
-        const bool act = line < a.nlines;
-        unsigned q = act ? (unsigned)line : 0u;
-        const unsigned i0 = q % n0;
-        q /= n0;
-        const unsigned i1 = q % n1, i2 = q / n1;
-        const unsigned bi = (i0 * (unsigned)a.ids[0] + i1 * (unsigned)a.ids[1] +
-                             i2 * (unsigned)a.ids[2]) * CS;
-        sfor<0, BPT0>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            const int b = u + i * TPL;
-            if ((NB0 % TPL == 0) || b < NB0) {
-                const unsigned vo = bi + (unsigned)b * isb;
-                sfor<0, R0>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-                    v[i][r] = buf_ld<R>(rin, vo, (unsigned)(r * NB0) * isb);
-                });
-            }
-        });
-        return act ? (i0 * (unsigned)a.ods[0] + i1 * (unsigned)a.ods[1] + i2 * (unsigned)a.ods[2]) * CS
-                   : 0xFFFFFFFFu;
-    };
-
-    /* one tile from its prefetched stage-0 inputs c */
-    auto process = [&](cx<R> (&c)[BPT0][R0], unsigned bo) {
-        /* stage 0 from registers into LDS (Ns = 1: outputs b*R0 + r) */
-        sfor<0, BPT0>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            const int b = u + i * TPL;
-            if ((NB0 % TPL == 0) || b < NB0) {
-                if (swi) {
-                    sfor<0, R0>([&](auto rc) {
-                        constexpr int r = decltype(rc)::value;
-                        c[i][r] = swp(c[i][r]);
-                    });
-                }
-                Dft<R, R0, 1>::run(c[i]);
-                sfor<0, R0>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-                    lds[G::idx(lt, b * R0 + r)] = c[i][r];
-                });
-            }
-        });
-        SyncLds::sync();
-        auto lds_ld = [&](int j) { return lds[G::idx(lt, j)]; };
-        auto lds_st = [&](int k, cx<R> w) { lds[G::idx(lt, k)] = w; };
-        sfor<1, RL::n - 1>([&](auto sc) {
-            stage<R, RL, decltype(sc)::value, L, TPL, true, true, SyncLds>(u, twl, lds_ld, lds_st);
-        });
-        const bool act = bo != 0xFFFFFFFFu;
-        auto gst = [&](int b, auto rc, cx<R> w) {
-            constexpr int r = decltype(rc)::value;
-            if (!act) return;
-            if (fl & FK_SCALE) { w.x *= (R)a.scale; w.y *= (R)a.scale; }
-            buf_st<R>(swo ? swp(w) : w, rout, bo + (unsigned)b * osb, (unsigned)(r * NS_LAST) * osb);
-        };
-        stage_last<R, RL, L, TPL>(u, twl, lds_ld, gst);
-        /* the next tile's stage 0 overwrites LDS the last stage just read */
-        SyncLds::sync();
-    };
-
-    /* two register sets, alternating: no copy between iterations */
-    cx<R> va[BPT0][R0], vb[BPT0][R0];
-    unsigned boa = tile < hi ? load(va, tile) : 0xFFFFFFFFu, bob = 0xFFFFFFFFu;
-    while (tile < hi) {
-        if (tile + g8 < hi) bob = load(vb, tile + g8);
-        process(va, boa);
-        tile += g8;
-        if (tile >= hi) break;
-        if (tile + g8 < hi) boa = load(va, tile + g8);
-        process(vb, bob);
-        tile += g8;
-    }
-}
-
-template <typename R, int NL, int MODE, int PAD, class RL, int TPL>
-constexpr unsigned stockham_loop_lds()
-{
-    using G = Geo<R, RL::L(), TPL, NL, MODE, PAD>;
-    return (G::lds_elems() + RL::twsize()) * sizeof(cx<R>);
 }
 
 } // namespace fk

@@ -52,9 +52,11 @@ static unsigned long long catalog_hash(void)
     if (!h) {
         int n, i;
         const fk_kernel *k = fk_registry(&n);
-        h = fk_fnv1a("fk-catalog-1", 12, 0);
+        /* the planner's decision sequence is part of what wisdom means:
+           the tag changes whenever decisions are added or reordered */
+        h = fk_fnv1a("fk-catalog-2/planner-chunk", 26, 0);
         for (i = 0; i < n; ++i) {
-            int v[20], j, m = 0;
+            int v[21], j, m = 0;
             v[m++] = k[i].prec;
             v[m++] = k[i].L;
             v[m++] = k[i].mode;
@@ -63,7 +65,7 @@ static unsigned long long catalog_hash(void)
             v[m++] = k[i].nl;
             v[m++] = k[i].variant;
             v[m++] = k[i].nrad;
-            for (j = 0; j < k[i].nrad && m < 20; ++j) v[m++] = k[i].rad[j];
+            for (j = 0; j < k[i].nrad && m < 21; ++j) v[m++] = k[i].rad[j];
             h = fk_fnv1a(v, sizeof(int) * m, h);
         }
     }

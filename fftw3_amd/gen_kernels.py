@@ -23,7 +23,6 @@ LDS_TARGET = 64 * 1024        # keep >= 2 workgroups per CU when possible
 CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
-LOOP_VARIANTS = False         # stockham_loop entries in the catalog
 
 
 def factorizations(n, maxr=16):
@@ -138,32 +137,7 @@ def variants(prec, L, rads, mode, io, g0):
             continue
         seen.add(key)
         out.append((prec, L, list(r), mode, io, g))
-    # persistent software-pipelined forms of the plain passes
-    if LOOP_VARIANTS and io == "PLAIN" and L <= 8192:
-        lcands = [(rads, 1, 1.0)] + ([(rads, 1, 2.0), (rads, 1, 0.5)] if mode == "COL" else [(rads, 1, 2.0)])
-        lseen = set()
-        for r, td, nm in lcands:
-            g = geometry(prec, L, r, mode, io, tpl_div=td, nl_mul=nm)
-            if not g or g["tpl"] < 16 or len(r) < 2:
-                continue
-            g = dict(g, loop=1, lds=g["lds"] - 8 * g["nl"] + twsize(r) * CSZ[prec])
-            if g["lds"] > LDS_MAX:
-                continue
-            key = (tuple(r), g["tpl"], g["nl"])
-            if key in lseen:
-                continue
-            lseen.add(key)
-            out.append((prec, L, list(r), mode, io, g))
     return out
-
-
-def twsize(rads):
-    """Complex entries of the per-stage twiddle table (RadList::twsize)."""
-    o, ns = 0, rads[0]
-    for r in rads[1:]:
-        o += ns * (r - 1)
-        ns *= r
-    return o
 
 
 def catalog():
@@ -201,10 +175,7 @@ def catalog():
 def cxx_entry(e):
     prec, L, rads, mode, io, g = e
     rl = "fk::RadList<%s>" % ",".join(map(str, rads))
-    if g.get("loop"):
-        targs = "%s,%d,FK_%s,%d,%s,%d" % (prec, g["nl"], mode, g["pad"], rl, g["tpl"])
-    else:
-        targs = "%s,%d,FK_%s,FK_IO_%s,%d,%s,%d" % (prec, g["nl"], mode, io, g["pad"], rl, g["tpl"])
+    targs = "%s,%d,FK_%s,FK_IO_%s,%d,%s,%d" % (prec, g["nl"], mode, io, g["pad"], rl, g["tpl"])
     return targs, rl
 
 
@@ -372,12 +343,10 @@ def main():
             prec, L, rads, mode, io, g, var = e
             targs, rl = cxx_entry(e[:6])
             radarr = ",".join(map(str, rads))
-            kname = "stockham_loop" if g.get("loop") else "stockham"
             lines.append(
-                '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::%s_lds<%s>(), '
-                '%s::twsize(), (const void *)&fk::%s<%s>, %d, %d},'
-                % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, kname, targs, rl,
-                   kname, targs, var, 1 if g.get("loop") else 0))
+                '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '
+                '%s::twsize(), (const void *)&fk::stockham<%s>, %d},'
+                % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, targs, rl, targs, var))
         lines += ['    };', '    *n = (int)(sizeof(tab) / sizeof(tab[0]));', '    return tab;', '}', '']
         path = os.path.join(OUT, "fk_inst_%d.hip" % p)
         written.add(path)

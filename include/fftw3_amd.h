@@ -68,6 +68,12 @@ int fftw_amd_plan_use_mfma(void *fftwf_plan);
 /* Planner flag for fftwf_ c2c plans: decompose power-of-16 lengths into
    16 / 256 / 4096-point passes and run them on the matrix-core variant. */
 #define FFTW_AMD_MFMA (1U << 24)
+/* Planner flag: plan multi-dimensional problems cache-blocked by default
+   (passes run per chunk of transforms, or per slab of planes for unbatched
+   rank >= 3, sized for the 256 MB memory-side cache), so intermediates are
+   re-read from that cache rather than HBM.  FFTW_MEASURE also times the
+   unblocked plan and keeps the faster. */
+#define FFTW_AMD_BLOCKED (1U << 25)
 /* Writes the 32x32 result of one probe matrix product (identity rows 0..7
    times a known B) to dev_out, for checking the operand layouts. */
 int fftw_amd_mfma_probe(float *dev_out);
