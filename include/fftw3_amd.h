@@ -71,7 +71,7 @@ int fftw_amd_plan_use_mfma(void *fftwf_plan);
 /* Planner flag: plan multi-dimensional problems cache-blocked by default
    (passes run per chunk of transforms, or per slab of planes for unbatched
    rank >= 3, sized for the 256 MB memory-side cache), so intermediates are
-   re-read from that cache rather than HBM.  FFTW_MEASURE also times the
+   re-read from that cache rather than HBM.  FFTW_PATIENT also times the
    unblocked plan and keeps the faster. */
 #define FFTW_AMD_BLOCKED (1U << 25)
 /* Writes the 32x32 result of one probe matrix product (identity rows 0..7
