@@ -253,8 +253,17 @@ __device__ __forceinline__ void cexp_fp64(long long m, long long n, double &c,
     s = -sn;
 }
 
+/* W_N^e in FP64 from the two-level table (see tw2) */
+__device__ __forceinline__ cx<double> tw2d(const fk_args &a, long long e);
+
 template <typename R>
 __device__ __forceinline__ cx<R> tw2(const fk_args &a, long long e)
+{
+    const cx<double> w = tw2d(a, e);
+    return mk((R)w.x, (R)w.y);
+}
+
+__device__ __forceinline__ cx<double> tw2d(const fk_args &a, long long e)
 {
     /* W_N^e from the two-level FP64 table, W_N^e = lo[e % b] * hi[e / b].
        Four-step exponents are e = k*i0 < N; the reduction below only runs
@@ -277,9 +286,7 @@ __device__ __forceinline__ cx<R> tw2(const fk_args &a, long long e)
         il = e - ih * b;
     }
     const double *lo = a.tlo + 2 * il, *hi = a.thi + 2 * ih;
-    double c = lo[0] * hi[0] - lo[1] * hi[1];
-    double s = lo[0] * hi[1] + lo[1] * hi[0];
-    return mk((R)c, (R)s);
+    return mk(lo[0] * hi[0] - lo[1] * hi[1], lo[0] * hi[1] + lo[1] * hi[0]);
 }
 
 } // namespace fk

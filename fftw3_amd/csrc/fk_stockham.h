@@ -205,9 +205,21 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         else return real ? lds[G::idx(lt, j)] : trash[tid & 63];
     };
 
-    /* output post-op: twiddle, scale, swap */
+    /* output post-op: twiddle, scale, swap.  The four-step twiddle of
+       output k = b + r*Ns (last stage, r = 0..R-1 stored in order) is
+       W_N^(k*i0): looked up from the FP64 table for r % 4 == 0 and
+       stepped by W_N^(Ns*i0) in FP64 in between (at most 3 products away
+       from a table value), so a thread gathers 2 table entries per 4
+       outputs instead of 2 per output. */
+    constexpr int NS_LAST = RL::ns(RL::n - 1);
+    cx<double> tw_cur = mk(1.0, 0.0), tw_step = mk(1.0, 0.0);
+    if (fl & FK_TW_OUT) tw_step = tw2d(a, (long long)NS_LAST * i0);
     auto post = [&](int k, cx<R> v) -> cx<R> {
-        if (fl & FK_TW_OUT) v = mul(v, tw2<R>(a, (long long)k * i0));
+        if (fl & FK_TW_OUT) {
+            if (((k / NS_LAST) & 3) == 0) tw_cur = tw2d(a, (long long)k * i0);
+            else tw_cur = mul(tw_cur, tw_step);
+            v = mul(v, mk((R)tw_cur.x, (R)tw_cur.y));
+        }
         if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
         return swo ? swp(v) : v;
     };

@@ -66,12 +66,17 @@ def geometry(prec, L, rads, mode, io, tpl_div=1, nl_mul=1.0):
         nl = max(1, 256 // tpl)
         while nl > 1 and nl * per_line > LDS_TARGET:
             nl //= 2
+        if io == "TOUT":
+            # transposed stores write runs of nl elements: aim for 128 B
+            # runs within the thread and LDS limits
+            while nl * csz < 128 and 2 * nl * tpl <= 1024 and 2 * nl * per_line + 16 * nl <= LDS_MAX:
+                nl *= 2
         if nl * per_line + 8 * nl > LDS_MAX:
             return None
     else:
         per_line = (L + 2) * csz
-        want = 16 if prec == "float" else 8
-        nl = want
+        # lines of a tile are adjacent in memory: aim for 256 B runs
+        nl = 256 // csz
         while nl > 1 and (nl * per_line > LDS_MAX - 1024 or nl * tpl > 1024):
             nl //= 2
         # small transforms: more lines per workgroup for occupancy
