@@ -267,17 +267,34 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         };
         auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
         run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
-        for (int k = u; k <= L; k += TPL) {
-            const cx<R> zk = lds_at(k == L ? 0 : k);
-            const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
-            const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
-            /* O = (Z[k] - conj Z[L-k]) / (2i) */
-            const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
-            const cx<R> w = wk[k];
-            cx<R> X = mk(er + orr * w.x - oi * w.y, ei + orr * w.y + oi * w.x);
-            if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
-            if (act) out[bout + k * a.os] = X;
-        }
+        /* bins k = u + i*TPL in groups of 8 (compile-time count): the
+           group's twiddle loads are in flight together */
+        constexpr int NK = (L + 1 + TPL - 1) / TPL;
+        constexpr int GK = 8;
+        sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
+            cx<R> w[cnt];
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if (k <= L) w[i] = wk[k];
+            });
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if (k <= L) {
+                    const cx<R> zk = lds_at(k == L ? 0 : k);
+                    const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
+                    const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
+                    /* O = (Z[k] - conj Z[L-k]) / (2i) */
+                    const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
+                    cx<R> X = mk(er + orr * w[i].x - oi * w[i].y, ei + orr * w[i].y + oi * w[i].x);
+                    if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
+                    if (act) out[bout + (long long)k * a.os] = X;
+                }
+            });
+        });
     } else if constexpr (IO == FK_IO_C2R) {
         /* half-length inverse: Z[k] = E[k] + i O[k],
            E[k] = X[k] + conj X[L-k], O[k] = (X[k] - conj X[L-k]) W_2L^-k,
