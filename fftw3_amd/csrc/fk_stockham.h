@@ -238,14 +238,17 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
         } else {
             auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
-            if (u == 0 && real) lbase[lt] = act ? bout : -1;
+            /* output offsets may be negative (negative strides): inactive
+               lines are marked with the one value no offset takes */
+            constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
+            if (u == 0 && real) lbase[lt] = act ? bout : NONE;
             run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
             /* lines-fastest store: consecutive threads write consecutive
                lines of the tile (unit output batch stride) */
             for (int e = tid; e < NL * L; e += NT) {
                 const int l2 = e % NL, k = e / NL;
                 const long long ob = lbase[l2];
-                if (ob >= 0) {
+                if (ob != NONE) {
                     cx<R> v = lds[G::idx(l2, k)];
                     if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
                     out[ob + k * a.os] = swo ? swp(v) : v;
