@@ -207,16 +207,19 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
 
     /* output post-op: twiddle, scale, swap.  The four-step twiddle of
        output k = b + r*Ns (last stage, r = 0..R-1 stored in order) is
-       W_N^(k*i0): looked up from the FP64 table for r % 4 == 0 and
-       stepped by W_N^(Ns*i0) in FP64 in between (at most 3 products away
-       from a table value), so a thread gathers 2 table entries per 4
-       outputs instead of 2 per output. */
+       W_N^(k*i0): looked up from the FP64 table every ANCH outputs and
+       stepped by W_N^(Ns*i0) in FP64 in between, so a thread gathers 2
+       table entries per ANCH outputs instead of 2 per output.  FP64 data
+       re-anchors every 4 (at most 3 products from a table value); FP32
+       data once per butterfly (at most 15 FP64 products, ~1e-15, far below
+       FP32 rounding). */
     constexpr int NS_LAST = RL::ns(RL::n - 1);
+    constexpr int ANCH = sizeof(R) == 8 ? 4 : 16;
     cx<double> tw_cur = mk(1.0, 0.0), tw_step = mk(1.0, 0.0);
     if (fl & FK_TW_OUT) tw_step = tw2d(a, (long long)NS_LAST * i0);
     auto post = [&](int k, cx<R> v) -> cx<R> {
         if (fl & FK_TW_OUT) {
-            if (((k / NS_LAST) & 3) == 0) tw_cur = tw2d(a, (long long)k * i0);
+            if ((k / NS_LAST) % ANCH == 0) tw_cur = tw2d(a, (long long)k * i0);
             else tw_cur = mul(tw_cur, tw_step);
             v = mul(v, mk((R)tw_cur.x, (R)tw_cur.y));
         }
