@@ -883,7 +883,7 @@ static int plan_c2c_dims(ctx *c, int rank, const fk_dim *dims, int vrank, const 
  * ESTIMATE and MEASURE plan the whole problem at once unless
  * FFTW_AMD_BLOCKED asks for blocking.
  */
-static const long long chunk_targets[] = {64LL << 20, 128LL << 20};
+static const long long chunk_targets[] = {64LL << 20, 128LL << 20, 192LL << 20};
 
 /* Offsets the arrays' byte offsets of passes [from, npass) (the chunk's
    position in the caller's arrays). */

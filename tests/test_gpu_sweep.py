@@ -289,3 +289,34 @@ def test_full_c4_2p24_sampled_bins(prec):
         err = np.max(np.abs(y[h, ks] - want)) / scale
         assert err <= (1e-13 if prec == "double" else 2e-6) * 10, (h, err)
     p.destroy()
+
+
+def test_beyond_2p31_elements():
+    """64-bit indexing: 2^19 + 3 transforms of 4096 FP32 points (just over
+    2^31 complex elements, 16 GiB); the first and the last transforms are
+    checked against the oracle, and a middle one by its DC bin."""
+    n, H = 4096, (1 << 19) + 3
+    tot = n * H
+    a = fa.DeviceBuffer(tot * 8)
+    b = fa.DeviceBuffer(tot * 8)
+    a.fill_uniform("float", 2 * tot, 6, 0)
+    p = fa.plan_many_dft("float", [n], H, a.ptr, None, 1, n, b.ptr, None, 1, n, -1)
+    p.execute()
+    for h in (0, H // 2, H - 1):
+        x = a.download(np.complex64, n, offset=h * n * 8).astype(np.complex128)
+        y = b.download(np.complex64, n, offset=h * n * 8).astype(np.complex128)
+        assert oracle.rel_l2(y, oracle.fftn(x, -1)) <= TOL["float"], h
+    p.destroy()
+    # the same arrays as 512 2-D problems of 2048^2 starting 12288 elements
+    # in, so the last one ends past element 2^31
+    sh = 12288
+    m = 2048 * 2048
+    q = fa.plan_many_dft("float", [2048, 2048], 512, a.ptr + sh * 8, None, 1, m, b.ptr + sh * 8, None, 1,
+                         m, +1)
+    q.execute()
+    last = sh + 511 * m
+    assert last + m > (1 << 31)
+    x = a.download(np.complex64, m, offset=last * 8).astype(np.complex128).reshape(2048, 2048)
+    y = b.download(np.complex64, m, offset=last * 8).astype(np.complex128).reshape(2048, 2048)
+    assert oracle.rel_l2(y, oracle.fftn(x, +1)) <= TOL["float"]
+    q.destroy()
