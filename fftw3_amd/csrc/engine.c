@@ -1804,7 +1804,7 @@ int fk_plan_use_mfma(fk_plan *p)
         ps->block = (unsigned)ps->k->tpl;
         ps->lds = 0;
         ps->grid = (unsigned)ps->a.nlines;
-        ps->a.xcd_remap = 0;
+        ps->a.xcd_remap = 1;
         ++n;
     }
     return n;

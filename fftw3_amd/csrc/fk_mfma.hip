@@ -33,6 +33,16 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 namespace {
 
+__device__ __forceinline__ long long fk_remap_line(unsigned xcd)
+{
+    long long w = blockIdx.x;
+    if (!xcd) return w;
+    /* workgroups are dispatched round-robin over 8 XCDs */
+    long long nwg = gridDim.x, per = nwg / 8, rem = nwg % 8;
+    long long x = w % 8, i = w / 8;
+    return x * per + (x < rem ? x : rem) + i;
+}
+
 /* 32x32 real DFT-16 matrix (row = output real index, col = input real index) */
 struct Dft16Real {
     float v[32][32];
@@ -68,7 +78,9 @@ __global__ void __launch_bounds__(64 * ((S == 3) ? 8 : 1)) k_mfma(fk_args a)
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = lane & 31, h = lane >> 5;
-    const long long line = blockIdx.x;
+    /* XCD-contiguous line order: neighbouring lines (which share cache
+       lines when the pass is strided) run on one XCD and its L2 */
+    const long long line = fk_remap_line(a.xcd_remap);
     const bool act = line < a.nlines;
     long long q = act ? line : 0;
     const long long i0 = q % a.nb[0];
