@@ -28,6 +28,10 @@ FFTW_PATIENT = 1 << 5
 FFTW_ESTIMATE = 1 << 6
 FFTW_WISDOM_ONLY = 1 << 21
 FFTW_AMD_MFMA = 1 << 24
+# r2r kinds (api/fftw3.h:89-100)
+FFTW_R2HC, FFTW_HC2R, FFTW_DHT = 0, 1, 2
+FFTW_REDFT00, FFTW_REDFT01, FFTW_REDFT10, FFTW_REDFT11 = 3, 4, 5, 6
+FFTW_RODFT00, FFTW_RODFT01, FFTW_RODFT10, FFTW_RODFT11 = 7, 8, 9, 10
 FFTW_AMD_BLOCKED = 1 << 25
 
 _lib = None
@@ -167,7 +171,14 @@ def lib():
         f("execute_split_dft", None, [vp, vp, vp, vp, vp])
         f("execute_split_dft_r2c", None, [vp, vp, vp, vp])
         f("execute_split_dft_c2r", None, [vp, vp, vp, vp])
-        f("plan_many_r2r", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, vp, u])
+        f("plan_many_r2r", vp, [ip, ipp, ip, vp, ipp, ip, ip, vp, ipp, ip, ip, ipp, u])
+        f("plan_r2r", vp, [ip, ipp, vp, vp, ipp, u])
+        f("plan_r2r_1d", vp, [ip, vp, vp, ip, u])
+        f("plan_r2r_2d", vp, [ip, ip, vp, vp, ip, ip, u])
+        f("plan_r2r_3d", vp, [ip, ip, ip, vp, vp, ip, ip, ip, u])
+        f("plan_guru_r2r", vp, [ip, ctypes.POINTER(IODim), ip, ctypes.POINTER(IODim), vp, vp, ipp, u])
+        f("plan_guru64_r2r", vp, [ip, ctypes.POINTER(IODim64), ip, ctypes.POINTER(IODim64), vp, vp, ipp, u])
+        f("execute_r2r", None, [vp, vp, vp])
         f("execute", None, [vp])
         f("execute_dft", None, [vp, vp, vp])
         f("execute_dft_r2c", None, [vp, vp, vp])
@@ -344,6 +355,9 @@ class Plan:
     def execute_c2r(self, i, o):
         fn(self.prec, "execute_dft_c2r")(self.h, i, o)
 
+    def execute_r2r(self, i, o):
+        fn(self.prec, "execute_r2r")(self.h, i, o)
+
     def execute_split(self, ri, ii, ro, io):
         fn(self.prec, "execute_split_dft")(self.h, ri, ii, ro, io)
 
@@ -460,6 +474,20 @@ def plan_many_dft_c2r(prec, n, howmany, inp, inembed, istride, idist, out, onemb
                       flags=FFTW_ESTIMATE):
     h = fn(prec, "plan_many_dft_c2r")(len(n), _iarr(n), howmany, inp, _iarr(inembed), istride, idist,
                                       out, _iarr(onembed), ostride, odist, flags)
+    return Plan(h, prec)
+
+
+def plan_many_r2r(prec, n, howmany, inp, inembed, istride, idist, out, onembed, ostride, odist, kinds,
+                  flags=FFTW_ESTIMATE):
+    h = fn(prec, "plan_many_r2r")(len(n), _iarr(n), howmany, inp, _iarr(inembed), istride, idist,
+                                  out, _iarr(onembed), ostride, odist, _iarr(kinds), flags)
+    return Plan(h, prec)
+
+
+def plan_guru_r2r(prec, dims, hdims, inp, out, kinds, flags=FFTW_ESTIMATE, wide=False):
+    name = "plan_guru64_r2r" if wide else "plan_guru_r2r"
+    h = fn(prec, name)(len(dims), _dims(dims, wide), len(hdims), _dims(hdims, wide), inp, out,
+                       _iarr(kinds), flags)
     return Plan(h, prec)
 
 

@@ -569,66 +569,102 @@ void X(execute_split_dft_c2r)(const PLAN p, R *ri, R *ii, R *out)
     }
 }
 
-/* ---------------- r2r (not provided: planners return NULL) ---------------- */
+/* ---------------- real-to-real kinds ----------------
+   api/plan-many-r2r.c:26-51, api/plan-guru-r2r.h:23-44, api/plan-r2r.c,
+   api/plan-r2r-{1,2,3}d.c; strides and dists count reals.  The kind of
+   each dim maps one to one (api/map-r2r-kind.c:24-50); an unknown kind
+   makes the planner return NULL. */
+
+static int r2r_kinds(fk_problem *pb, int rank, const X(r2r_kind) *kind)
+{
+    int i;
+    if (rank > 0 && !kind) return 0;
+    for (i = 0; i < rank; ++i) {
+        if ((int)kind[i] < FFTW_R2HC || (int)kind[i] > FFTW_RODFT11) return 0;
+        pb->r2r[i] = (int)kind[i];
+    }
+    return 1;
+}
 
 PLAN X(plan_many_r2r)(int rank, const int *n, int howmany, R *in, const int *inembed,
                       int istride, int idist, R *out, const int *onembed, int ostride, int odist,
                       const X(r2r_kind) *kind, unsigned flags)
 {
-    (void)rank; (void)n; (void)howmany; (void)in; (void)inembed; (void)istride; (void)idist;
-    (void)out; (void)onembed; (void)ostride; (void)odist; (void)kind; (void)flags;
-    return 0;
+    fk_problem pb;
+    if (!many_kosher(rank, n, howmany)) return 0;
+    memset(&pb, 0, sizeof(pb));
+    if (!r2r_kinds(&pb, rank, kind)) return 0;
+    pb.kind = FK_R2R;
+    pb.flags = flags;
+    pb.rank = rank;
+    rowmajor(rank, n, inembed ? inembed : n, onembed ? onembed : n, istride, ostride, pb.dims);
+    pb.vrank = 1;
+    pb.vdims[0].n = howmany;
+    pb.vdims[0].is = idist;
+    pb.vdims[0].os = odist;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
 }
 
 PLAN X(plan_r2r)(int rank, const int *n, R *in, R *out, const X(r2r_kind) *kind,
                  unsigned flags)
 {
-    (void)rank; (void)n; (void)in; (void)out; (void)kind; (void)flags;
-    return 0;
+    return X(plan_many_r2r)(rank, n, 1, in, 0, 1, 1, out, 0, 1, 1, kind, flags);
 }
 
 PLAN X(plan_r2r_1d)(int n, R *in, R *out, X(r2r_kind) kind, unsigned flags)
 {
-    (void)n; (void)in; (void)out; (void)kind; (void)flags;
-    return 0;
+    return X(plan_r2r)(1, &n, in, out, &kind, flags);
 }
 
 PLAN X(plan_r2r_2d)(int n0, int n1, R *in, R *out, X(r2r_kind) kind0, X(r2r_kind) kind1,
                     unsigned flags)
 {
-    (void)n0; (void)n1; (void)in; (void)out; (void)kind0; (void)kind1; (void)flags;
-    return 0;
+    int n[2] = {n0, n1};
+    X(r2r_kind) k[2] = {kind0, kind1};
+    return X(plan_r2r)(2, n, in, out, k, flags);
 }
 
 PLAN X(plan_r2r_3d)(int n0, int n1, int n2, R *in, R *out, X(r2r_kind) kind0,
                     X(r2r_kind) kind1, X(r2r_kind) kind2, unsigned flags)
 {
-    (void)n0; (void)n1; (void)n2; (void)in; (void)out; (void)kind0; (void)kind1; (void)kind2;
-    (void)flags;
-    return 0;
+    int n[3] = {n0, n1, n2};
+    X(r2r_kind) k[3] = {kind0, kind1, kind2};
+    return X(plan_r2r)(3, n, in, out, k, flags);
 }
 
 PLAN X(plan_guru_r2r)(int rank, const X(iodim) *dims, int howmany_rank,
                       const X(iodim) *howmany_dims, R *in, R *out, const X(r2r_kind) *kind,
                       unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims; (void)in; (void)out;
-    (void)kind; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim))
+    if (!r2r_kinds(&pb, rank, kind)) return 0;
+    pb.kind = FK_R2R;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
 }
 
 PLAN X(plan_guru64_r2r)(int rank, const X(iodim64) *dims, int howmany_rank,
                         const X(iodim64) *howmany_dims, R *in, R *out, const X(r2r_kind) *kind,
                         unsigned flags)
 {
-    (void)rank; (void)dims; (void)howmany_rank; (void)howmany_dims; (void)in; (void)out;
-    (void)kind; (void)flags;
-    return 0;
+    fk_problem pb;
+    GURU_BODY(X(iodim64))
+    if (!r2r_kinds(&pb, rank, kind)) return 0;
+    pb.kind = FK_R2R;
+    pb.in = in;
+    pb.out = out;
+    return mk(&pb);
 }
 
+/* api/execute-r2r.c:24-28: new arrays, same layout and alignment */
 void X(execute_r2r)(const PLAN p, R *in, R *out)
 {
-    (void)p; (void)in; (void)out;
+    if (!p) return;
+    fk_execute_or_die(P_(p), in, out);
 }
 
 /* ---------------- plan lifetime (api/apiplan.c:180-210) ---------------- */

@@ -1261,6 +1261,267 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
 }
 
 /* ------------------------------------------------------------------ */
+/* real-to-real kinds                                                  */
+
+/* Each dimension of an r2r problem is one pass group: a pre pass from the
+   source into a contiguous staging line, an r2c / c2r / c2c of that line,
+   and a post pass into the destination with its strides.  What each kind
+   computes is doc/reference.texi:2021-2350 (DCT-I..IV, DST-I..IV, DHT) and
+   :913-930 (halfcomplex); the staging algorithms are:
+     R2HC, DHT       n-point r2c straight from the source, then the
+                     halfcomplex / Hartley post pass;
+     HC2R            Hermitian bins from halfcomplex, n-point c2r straight
+                     into the destination;
+     REDFT10/RODFT10 reordering (Makhoul), n-point r2c, post twiddle
+                     exp(-i pi k / 2n) (FFTW: reodft/reodft010e-r2hc.c);
+     REDFT01/RODFT01 pre twiddle to n/2+1 bins, n-point c2r, reordering;
+     REDFT11/RODFT11 even n: n/2-point c2c of (x_2m + i x_{n-1-2m}) with pre
+                     and post twiddles; odd n: 2n-point c2c of the
+                     twiddled, zero-padded line;
+     REDFT00/RODFT00 r2c of the even / odd extension of logical size
+                     2(n-1) / 2(n+1) (reodft/redft00e-r2hc-pad.c,
+                     reodft/rodft00e-r2hc-pad.c).
+   The sine kinds reverse or negate samples around the cosine algorithms. */
+
+static int r2r_trivial(int kind, long long n)
+{
+    /* rdft/problem.c:129-133: size-1 dims of these kinds are the identity */
+    return n == 1 && (kind == FK_K_R2HC || kind == FK_K_HC2R || kind == FK_K_DHT ||
+                      kind == FK_K_REDFT01 || kind == FK_K_RODFT01);
+}
+
+static const char *r2r_name(int kind)
+{
+    static const char *nm[] = {"r2hc", "hc2r", "dht", "e00", "e01", "e10", "e11",
+                               "o00", "o01", "o10", "o11"};
+    return kind >= 0 && kind <= FK_K_RODFT11 ? nm[kind] : "?";
+}
+
+static void emit_r2r_ew(ctx *c, int op, long long n, int sine, long long cnt, long long is,
+                        long long os, int nb, const fk_dim *b, int ib, int ob, size_t isz,
+                        size_t osz, const void *wt, double bytes)
+{
+    fk_pass proto;
+    memset(&proto, 0, sizeof(proto));
+    proto.type = FK_PASS_EW;
+    proto.e.op = op;
+    proto.e.n = cnt;
+    proto.e.is = is;
+    proto.e.os = os;
+    proto.e.p0 = n;
+    proto.e.p1 = sine;
+    proto.e.aux = wt;
+    proto.ibuf = ib;
+    proto.obuf = ob;
+    proto.bytes = bytes;
+    emit_loops(c, &proto, nb, b, isz, osz);
+}
+
+/* rank-1 r2c (kind FK_R2C) or c2r over a batch, between buffers ib / ob */
+static int plan_real_line(ctx *c, int kind, long long n, long long is, long long os, int nb,
+                          const fk_dim *b, int ib, void *iptr, int ob, void *optr)
+{
+    fk_problem q;
+    int sbin = c->bin, sbout = c->bout, rc;
+    unsigned align = c->P->align_need;
+    memset(&q, 0, sizeof(q));
+    q.kind = kind;
+    q.prec = c->P->prec;
+    q.sign = kind == FK_R2C ? -1 : 1;
+    q.rank = 1;
+    q.dims[0].n = n;
+    q.dims[0].is = is;
+    q.dims[0].os = os;
+    q.vrank = nb;
+    memcpy(q.vdims, b, sizeof(fk_dim) * nb);
+    q.in = iptr;
+    q.out = optr;
+    c->bin = ib;
+    c->bout = ob;
+    rc = kind == FK_R2C ? plan_r2c(c, &q) : plan_c2r(c, &q);
+    c->bin = sbin;
+    c->bout = sbout;
+    /* a real side in staging memory puts no requirement on the user's arrays */
+    if (!(kind == FK_R2C ? iptr : optr)) c->P->align_need = align;
+    return rc;
+}
+
+static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os, int nb0,
+                       const fk_dim *b0, int ib, void *iptr, int ob, void *optr)
+{
+    fk_plan *P = c->P;
+    int prec = P->prec, nb, i, sine, pre = -1, post = -1, inner, acplx, bcplx;
+    size_t cs = csz(prec), rs = rsz(prec), as, bs;
+    fk_dim b[FK_MAXRANK * 2], pa[FK_MAXRANK * 2], in_[FK_MAXRANK * 2], pb_[FK_MAXRANK * 2];
+    long long H, la = 0, lb = 0, M = n, sa, sb;
+    const void *wt = 0;
+    const int SA = FK_BUF_S0 + FK_R2R_SLOT, SB = FK_BUF_S0 + FK_R2R_SLOT + 1;
+    memcpy(b, b0, sizeof(fk_dim) * nb0);
+    nb = squeeze(nb0, b);
+    sort_by_is(nb, b);
+    H = batch_count(nb, b);
+    sine = kind >= FK_K_RODFT00;
+    if (n == 1) {
+        /* size-1 dims with a constant factor: REDFT10 / RODFT10 / RODFT00
+           give 2 x_0, REDFT11 / RODFT11 give sqrt(2) x_0 */
+        fk_pass proto;
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_COPY_R;
+        proto.e.n = 1;
+        proto.e.is = is;
+        proto.e.os = os;
+        proto.e.flags = FK_SCALE;
+        proto.e.scale = (kind == FK_K_REDFT11 || kind == FK_K_RODFT11) ? sqrt(2.0) : 2.0;
+        proto.ibuf = ib;
+        proto.obuf = ob;
+        proto.bytes = 2.0 * rs * (double)H;
+        emit_loops(c, &proto, nb, b, rs, rs);
+        return c->err ? -1 : 0;
+    }
+    switch (kind) {
+    case FK_K_R2HC:
+    case FK_K_DHT:
+        inner = FK_R2C; post = FK_EW_HC_POST; sine = kind == FK_K_DHT;
+        acplx = 0; bcplx = 1; lb = n / 2 + 1;
+        break;
+    case FK_K_HC2R:
+        inner = FK_C2R; pre = FK_EW_HC_PRE;
+        acplx = 1; bcplx = 0; la = n / 2 + 1;
+        break;
+    case FK_K_REDFT10:
+    case FK_K_RODFT10:
+        inner = FK_R2C; pre = FK_EW_DCT2_PRE; post = FK_EW_DCT2_POST;
+        acplx = 0; bcplx = 1; la = n; lb = n / 2 + 1;
+        break;
+    case FK_K_REDFT01:
+    case FK_K_RODFT01:
+        inner = FK_C2R; pre = FK_EW_DCT3_PRE; post = FK_EW_DCT3_POST;
+        acplx = 1; bcplx = 0; la = n / 2 + 1; lb = n;
+        break;
+    case FK_K_REDFT11:
+    case FK_K_RODFT11:
+        inner = FK_C2C; acplx = bcplx = 1;
+        if (n % 2 == 0) {
+            pre = FK_EW_DCT4_PRE; post = FK_EW_DCT4_POST; M = la = lb = n / 2;
+        } else {
+            pre = FK_EW_DCT4O_PRE; post = FK_EW_DCT4O_POST; M = la = lb = 2 * n;
+        }
+        break;
+    case FK_K_REDFT00:
+    case FK_K_RODFT00:
+        inner = FK_R2C; pre = FK_EW_SYM_PRE; post = FK_EW_SYM_POST;
+        M = la = kind == FK_K_REDFT00 ? 2 * (n - 1) : 2 * (n + 1);
+        acplx = 0; bcplx = 1; lb = M / 2 + 1;
+        break;
+    default:
+        set_err("invalid r2r kind %d", kind);
+        return -1;
+    }
+    as = acplx ? cs : rs;
+    bs = bcplx ? cs : rs;
+    if (pre == FK_EW_DCT2_PRE || pre == FK_EW_DCT3_PRE || pre == FK_EW_DCT4_PRE ||
+        pre == FK_EW_DCT4O_PRE) {
+        wt = line_table(prec, 8 * n, 2 * n, c->st);
+        if (!wt) { c->err = 1; set_err("twiddle table allocation failed"); return -1; }
+    }
+    /* staging layouts: lines contiguous, batch dims in the order of b */
+    sa = la;
+    sb = lb;
+    for (i = 0; i < nb; ++i) {
+        pa[i].n = in_[i].n = pb_[i].n = b[i].n;
+        pa[i].is = b[i].is;
+        pa[i].os = sa;
+        in_[i].is = sa;
+        in_[i].os = sb;
+        pb_[i].is = sb;
+        pb_[i].os = b[i].os;
+        sa *= b[i].n;
+        sb *= b[i].n;
+    }
+    if (pre >= 0) {
+        need_scratch(c, FK_R2R_SLOT, as * (size_t)la * (size_t)H);
+        emit_r2r_ew(c, pre, n, sine, la, is, 1, nb, pa, ib, SA, rs, as, wt,
+                    (double)H * ((double)n * rs + (double)la * as));
+    } else {
+        /* the inner transform reads the source directly */
+        for (i = 0; i < nb; ++i) in_[i].is = b[i].is;
+    }
+    if (post >= 0) {
+        need_scratch(c, FK_R2R_SLOT + 1, bs * (size_t)lb * (size_t)H);
+    } else {
+        for (i = 0; i < nb; ++i) in_[i].os = b[i].os;
+    }
+    if (c->err) return -1;
+    if (inner == FK_C2C) {
+        if (plan_c2c_1d(c, M, 1, 1, nb, in_, SA, 0, SB, 0, 0, 0, 0)) return -1;
+    } else {
+        int iib = pre >= 0 ? SA : ib, oob = post >= 0 ? SB : ob;
+        if (plan_real_line(c, inner, M, pre >= 0 ? 1 : is, post >= 0 ? 1 : os, nb, in_, iib,
+                           pre >= 0 ? 0 : iptr, oob, post >= 0 ? 0 : optr))
+            return -1;
+    }
+    if (post >= 0)
+        emit_r2r_ew(c, post, n, sine, n, 1, os, nb, pb_, SB, ob, bs, rs, wt,
+                    (double)H * ((double)lb * bs + (double)n * rs));
+    return c->err ? -1 : 0;
+}
+
+/* Multi-dimensional r2r: the separable product of the 1-D kinds
+   (doc/reference.texi:2352-2358), one dimension after another from the
+   last; the first group reads the input, the others work in place on the
+   output. */
+static int plan_r2r(ctx *c, const fk_problem *pb)
+{
+    fk_plan *P = c->P;
+    fk_dim d[FK_MAXRANK];
+    int kd[FK_MAXRANK], r = 0, i, s;
+    for (i = 0; i < pb->rank; ++i) {
+        if (pb->r2r[i] < 0 || pb->r2r[i] > FK_K_RODFT11) {
+            set_err("invalid r2r kind %d", pb->r2r[i]);
+            return -1;
+        }
+        if (pb->r2r[i] == FK_K_REDFT00 && pb->dims[i].n == 1) {
+            /* doc/reference.texi:2083: not defined for n = 1 */
+            set_err("REDFT00 of size 1 is not defined");
+            return -1;
+        }
+    }
+    desc_add(P, "(r2r rank=%d", pb->rank);
+    for (i = 0; i < pb->rank; ++i) desc_add(P, " %s%lld", r2r_name(pb->r2r[i]), pb->dims[i].n);
+    desc_add(P, ")");
+    for (i = 0; i < pb->rank; ++i) {
+        if (r2r_trivial(pb->r2r[i], pb->dims[i].n)) continue;
+        d[r] = pb->dims[i];
+        kd[r++] = pb->r2r[i];
+    }
+    if (r == 0) {
+        if (!P->inplace) plan_copy(c, pb->vrank, pb->vdims, FK_EW_COPY_R, rsz(P->prec));
+        return c->err ? -1 : 0;
+    }
+    for (s = 0; s < r; ++s) {
+        int dd = r - 1 - s, nb = 0, first = s == 0;
+        fk_dim b[FK_MAXRANK * 2];
+        for (i = 0; i < r; ++i) {
+            if (i == dd) continue;
+            b[nb] = d[i];
+            if (!first) b[nb].is = b[nb].os;
+            ++nb;
+        }
+        for (i = 0; i < pb->vrank; ++i) {
+            b[nb] = pb->vdims[i];
+            if (!first) b[nb].is = b[nb].os;
+            ++nb;
+        }
+        if (plan_r2r_1d(c, kd[dd], d[dd].n, first ? d[dd].is : d[dd].os, d[dd].os, nb, b,
+                        first ? FK_BUF_IN : FK_BUF_OUT, first ? pb->in : pb->out, FK_BUF_OUT,
+                        pb->out))
+            return -1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* validation and normalisation                                        */
 
 static int dims_kosher(const fk_problem *pb)
@@ -1508,6 +1769,15 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
         } else if ((rb = plan_real_blocked(&c, &pb)) < 0 || (!rb && plan_r2c(&c, &pb))) {
             goto fail;
         }
+    } else if (pb.kind == FK_R2R) {
+        P->inplace = pb.in == pb.out;
+        memcpy(P->r2r, pb.r2r, sizeof(int) * pb.rank);
+        if (P->inplace && !inplace_ok(&pb)) goto fail;
+        if (((uintptr_t)pb.in | (uintptr_t)pb.out) % rsz(pb.prec)) {
+            set_err("arrays must be aligned to the real element size");
+            goto fail;
+        }
+        if (plan_r2r(&c, &pb)) goto fail;
     } else {
         P->inplace = pb.in == pb.out;
         if (pb.rank > 1 && (pb.flags & FK_PRESERVE_INPUT)) {
@@ -1563,7 +1833,7 @@ fail:
    kernel/planner.c:170-177) */
 static unsigned long long problem_hash(const fk_problem *pb)
 {
-    long long v[8 + 6 * FK_MAXRANK];
+    long long v[8 + 7 * FK_MAXRANK];
     int m = 0, i;
     v[m++] = pb->kind;
     v[m++] = pb->prec;
@@ -1578,6 +1848,7 @@ static unsigned long long problem_hash(const fk_problem *pb)
         v[m++] = pb->dims[i].n;
         v[m++] = pb->dims[i].is;
         v[m++] = pb->dims[i].os;
+        if (pb->kind == FK_R2R) v[m++] = pb->r2r[i];
     }
     for (i = 0; i < pb->vrank; ++i) {
         v[m++] = pb->vdims[i].n;

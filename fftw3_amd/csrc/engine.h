@@ -22,10 +22,16 @@ extern "C" {
 #endif
 
 #define FK_MAXRANK 16
-#define FK_MAX_SCRATCH 9
+#define FK_MAX_SCRATCH 11
 #define FK_SPLIT_SLOT 8         /* interleaved staging of split-array plans */
+#define FK_R2R_SLOT 9           /* r2r staging: slot 9 pre-pass output, 10 inner output */
 
-enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2 };
+enum { FK_C2C = 0, FK_R2C = 1, FK_C2R = 2, FK_R2R = 3 };
+/* r2r kinds per dimension, same values as fftw3.h's fftw_r2r_kind
+   (api/fftw3.h:89-100) */
+enum { FK_K_R2HC = 0, FK_K_HC2R = 1, FK_K_DHT = 2,
+       FK_K_REDFT00 = 3, FK_K_REDFT01 = 4, FK_K_REDFT10 = 5, FK_K_REDFT11 = 6,
+       FK_K_RODFT00 = 7, FK_K_RODFT01 = 8, FK_K_RODFT10 = 9, FK_K_RODFT11 = 10 };
 /* symbolic buffers of a pass: the arrays given to execute (IN2 / OUT2 are
    the imaginary arrays of split-complex plans) and the plan's scratch */
 enum { FK_BUF_IN = 0, FK_BUF_OUT = 1, FK_BUF_IN2 = 2, FK_BUF_OUT2 = 3, FK_BUF_S0 = 4 };
@@ -76,6 +82,7 @@ typedef struct fk_plan {
     void *in, *out;             /* arrays given at planning */
     int split;                  /* FK_SPLIT_* */
     void *in2, *out2;           /* imaginary arrays at planning (FK_SPLIT_GENERAL) */
+    int r2r[FK_MAXRANK];        /* FK_R2R: kind of each dim */
     int npass, cpass;
     fk_pass *pass;
     int nscratch;
@@ -105,6 +112,8 @@ typedef struct {
        reals; in2 / out2 are the imaginary arrays of the complex side(s) */
     int split;
     void *in2, *out2;
+    /* FK_R2R: kind (FK_K_*) of each transform dim; strides count reals */
+    int r2r[FK_MAXRANK];
 } fk_problem;
 
 /* ---- engine entry points ---- */

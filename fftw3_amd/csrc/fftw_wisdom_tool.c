@@ -8,10 +8,14 @@
  * (tools/fftw-wisdom.c:72-130 options and help, :134-147 canonical sizes,
  * :291-330 problem list and time limit) and libbench2's problem strings
  * (libbench2/problem.c:229-318):
- *   <problem>  = <prefix>* <n1>[x<n2>...] [*<howmany>]
+ *   <problem>  = <prefix>* <n1>[<kind>][x<n2>[<kind>]...] [*<howmany>]
  *   <prefix>   = c (complex) | r (real: r2c forward, c2r backward)
- *              | i (in place) | o (out of place) | f / - (forward)
- *              | b / + (backward) | / (split real and imaginary arrays)
+ *              | k (real-to-real) | i (in place) | o (out of place)
+ *              | f / - (forward) | b / + (backward)
+ *              | / (split real and imaginary arrays)
+ *   <kind>     = r2r kind of the preceding dim (libbench2/problem.c:152-195):
+ *                f R2HC (default), b HC2R, h DHT, e00/e01/e10/e11 REDFTab,
+ *                o00/o01/o10/o11 RODFTab
  * The engine keeps both precisions in one wisdom store; -s (or running the
  * program as fftwf-wisdom) plans with the fftwf_ planners.
  *
@@ -32,8 +36,9 @@
 
 typedef struct {
     char text[64];
-    int real, inplace, sign, split;
+    int real, inplace, sign, split, r2r;
     int rank, n[MAXR];
+    int kind[MAXR];            /* r2r kinds (fftw_r2r_kind) */
     int howmany;
     double size;               /* points, for ordering */
 } problem;
@@ -60,8 +65,9 @@ static int parse_problem(const char *s, problem *p)
     p->howmany = 1;
     for (;; ++q) {
         int ch = tolower((unsigned char)*q);
-        if (ch == 'c') p->real = 0;
-        else if (ch == 'r') p->real = 1;
+        if (ch == 'c') p->real = p->r2r = 0;
+        else if (ch == 'r') { p->real = 1; p->r2r = 0; }
+        else if (ch == 'k') { p->r2r = 1; p->real = 0; }
         else if (ch == 'i') p->inplace = 1;
         else if (ch == 'o') p->inplace = 0;
         else if (ch == 'f' || ch == '-') p->sign = -1;
@@ -74,8 +80,24 @@ static int parse_problem(const char *s, problem *p)
         if (!isdigit((unsigned char)*q)) return -1;
         v = strtol(q, &end, 10);
         if (v <= 0 || v > (1L << 30)) return -1;
-        p->n[p->rank++] = (int)v;
+        p->n[p->rank] = (int)v;
+        p->kind[p->rank++] = FFTW_R2HC;
         q = end;
+        if (p->r2r) {
+            int ch = tolower((unsigned char)*q);
+            if (ch == 'f') { ++q; }
+            else if (ch == 'b') { p->kind[p->rank - 1] = FFTW_HC2R; ++q; }
+            else if (ch == 'h') { p->kind[p->rank - 1] = FFTW_DHT; ++q; }
+            else if (ch == 'e' || ch == 'o') {
+                static const char *ab[] = {"00", "01", "10", "11"};
+                int t, hit = -1;
+                for (t = 0; t < 4; ++t)
+                    if (!strncmp(q + 1, ab[t], 2) && !isdigit((unsigned char)q[3])) hit = t;
+                if (hit < 0) return -1;
+                p->kind[p->rank - 1] = (ch == 'e' ? FFTW_REDFT00 : FFTW_RODFT00) + hit;
+                q += 3;
+            }
+        }
         if (*q == 'x' || *q == 'X') {
             ++q;
             continue;
@@ -90,8 +112,8 @@ static int parse_problem(const char *s, problem *p)
         p->howmany = (int)v;
         q = end;
     }
-    if (*q) return -1; /* trailing characters: strides, r2r kinds, ... */
-    if (p->split && p->real) return -1;
+    if (*q) return -1; /* trailing characters: strides, ... */
+    if (p->split && (p->real || p->r2r)) return -1;
     p->size = p->howmany;
     for (v = 0; v < p->rank; ++v) p->size *= p->n[v];
     return 0;
@@ -113,7 +135,11 @@ static int plan_problem(const problem *p, int single, unsigned flags)
     for (i = 0; i < p->rank; ++i) N *= p->n[i];
     Nh = N / p->n[p->rank - 1] * (p->n[p->rank - 1] / 2 + 1);
     padded = 2 * Nh;
-    if (!p->real) {
+    if (p->r2r) {
+        size_t bytes = (size_t)N * p->howmany * rs;
+        in = fftw_malloc(bytes);
+        out = p->inplace ? in : fftw_malloc(bytes);
+    } else if (!p->real) {
         size_t bytes = (size_t)N * p->howmany * (p->split ? rs : cs);
         in = fftw_malloc(bytes);
         out = p->inplace ? in : fftw_malloc(bytes);
@@ -130,7 +156,14 @@ static int plan_problem(const problem *p, int single, unsigned flags)
     if (!in || !out || (p->split && (!in2 || !out2))) goto done;
     dist_r = (int)(p->inplace ? padded : N);
     dist_c = (int)Nh;
-    if (p->split) {
+    if (p->r2r) {
+        if (single)
+            plan = fftwf_plan_many_r2r(p->rank, p->n, p->howmany, in, 0, 1, (int)N, out, 0, 1,
+                                       (int)N, (const fftwf_r2r_kind *)p->kind, flags);
+        else
+            plan = fftw_plan_many_r2r(p->rank, p->n, p->howmany, in, 0, 1, (int)N, out, 0, 1,
+                                      (int)N, (const fftw_r2r_kind *)p->kind, flags);
+    } else if (p->split) {
         fftw_iodim dims[MAXR], hd;
         long long s = 1;
         for (i = p->rank - 1; i >= 0; --i) {
@@ -210,11 +243,14 @@ static void help(FILE *f, const char *prog)
             "                 -s, --single: plan single-precision (fftwf_) transforms\n"
             "                   -l, --list: print the parsed problem list and exit\n"
             "\nSize syntax: <type><inplace><direction><geometry>[*<howmany>]\n"
-            "      <type> = c/r for complex/real (r2c forward, c2r backward); a '/'\n"
-            "               prefix asks for split real and imaginary arrays\n"
+            "      <type> = c/r/k for complex/real (r2c forward, c2r backward)/r2r;\n"
+            "               a '/' prefix asks for split real and imaginary arrays\n"
             "   <inplace> = i/o for in/out-of place\n"
-            " <direction> = f/b for forward/backward\n"
-            "  <geometry> = <n1>[x<n2>[x...]], e.g. 10x12x14\n",
+            " <direction> = f/b for forward/backward, omitted for k transforms\n"
+            "  <geometry> = <n1>[x<n2>[x...]], e.g. 10x12x14\n"
+            "               -- for k transforms, after each dimension is a <kind>:\n"
+            "                     <kind> = f/b/h/e00/e01/e10/e11/o00/o01/o10/o11\n"
+            "                              for R2HC/HC2R/DHT/REDFT00/.../RODFT11\n",
             prog);
 }
 
@@ -306,11 +342,22 @@ int main(int argc, char **argv)
 
     if (list) {
         for (i = 0; i < np; ++i) {
-            printf("%s %s %s %s%s rank=%d n=", probs[i].text, probs[i].real ? "real" : "complex",
-                   probs[i].inplace ? "in-place" : "out-of-place",
-                   probs[i].sign < 0 ? "forward" : "backward", probs[i].split ? " split" : "",
-                   probs[i].rank);
+            static const char *kn[] = {"r2hc", "hc2r", "dht", "redft00", "redft01", "redft10",
+                                       "redft11", "rodft00", "rodft01", "rodft10", "rodft11"};
+            if (probs[i].r2r)
+                printf("%s r2r %s rank=%d n=", probs[i].text,
+                       probs[i].inplace ? "in-place" : "out-of-place", probs[i].rank);
+            else
+                printf("%s %s %s %s%s rank=%d n=", probs[i].text,
+                       probs[i].real ? "real" : "complex",
+                       probs[i].inplace ? "in-place" : "out-of-place",
+                       probs[i].sign < 0 ? "forward" : "backward", probs[i].split ? " split" : "",
+                       probs[i].rank);
             for (j = 0; j < probs[i].rank; ++j) printf("%s%d", j ? "x" : "", probs[i].n[j]);
+            if (probs[i].r2r) {
+                printf(" kind=");
+                for (j = 0; j < probs[i].rank; ++j) printf("%s%s", j ? "," : "", kn[probs[i].kind[j]]);
+            }
             printf(" howmany=%d\n", probs[i].howmany);
         }
         free(probs);

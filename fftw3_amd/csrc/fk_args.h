@@ -81,7 +81,22 @@ enum {
     FK_EW_HERM,         /* n/2+1 bins -> full Hermitian line of n = p0 */
     FK_EW_REAL,         /* complex -> real part */
     FK_EW_SPLIT_GATHER, /* (in[j], in2[j]) reals -> interleaved complex */
-    FK_EW_SPLIT_SCATTER /* interleaved complex -> out[j] (re), out2[j] (im) */
+    FK_EW_SPLIT_SCATTER,/* interleaved complex -> out[j] (re), out2[j] (im) */
+    /* r2r pre / post passes around an r2c, c2r or c2c of a staging line.
+       p0 = n (the r2r length), p1 = 1 for the sine kinds; aux = the table
+       exp(-i pi m / 4n), m < 2n */
+    FK_EW_HC_POST,      /* n/2+1 bins -> halfcomplex (p1 = 0) or DHT (p1 = 1) */
+    FK_EW_HC_PRE,       /* halfcomplex -> n/2+1 Hermitian bins */
+    FK_EW_DCT2_PRE,     /* x -> v (even samples up, odd samples down) */
+    FK_EW_DCT2_POST,    /* V (n/2+1 bins) -> 2 Re(w^k V_k) */
+    FK_EW_DCT3_PRE,     /* x -> conj(w^k) (x_k - i x_{n-k}), k <= n/2 */
+    FK_EW_DCT3_POST,    /* v (n reals) -> interleave v_j, v_{n-1-j} */
+    FK_EW_DCT4_PRE,     /* even n: (x_2m + i x_{n-1-2m}) w^{4m+1} */
+    FK_EW_DCT4_POST,    /* even n: Y_2p = 2 Re(c_p), Y_{n-1-2p} = -2 Im(c_p) */
+    FK_EW_DCT4O_PRE,    /* odd n: x_j w^{2j}, zero up to 2n */
+    FK_EW_DCT4O_POST,   /* odd n: Y_k = 2 Re(w^{2k+1} A_k) */
+    FK_EW_SYM_PRE,      /* even (p1 = 0, length 2(n-1)) / odd (p1 = 1, 2(n+1)) extension */
+    FK_EW_SYM_POST      /* Re(B_k) / -Im(B_{k+1}) */
 };
 
 typedef struct {

@@ -172,9 +172,12 @@ __global__ void k_ew(fk_ew_args a)
         co[j * a.os] = fin(v);
         break;
     }
-    case FK_EW_COPY_R:
-        ro[j * a.os] = ri[j * a.is];
+    case FK_EW_COPY_R: {
+        R v = ri[j * a.is];
+        if (a.flags & FK_SCALE) v *= (R)a.scale;
+        ro[j * a.os] = v;
         break;
+    }
     case FK_EW_SPLIT_GATHER: {
         const R *ri2 = reinterpret_cast<const R *>(a.in2) + bi;
         co[j * a.os] = mk(ri[j * a.is], ri2[j * a.is]);
@@ -253,6 +256,131 @@ __global__ void k_ew(fk_ew_args a)
         ro[j * a.os] = v;
         break;
     }
+    /* r2r pre / post passes; n = p0, sine kinds p1 = 1, aux[m] = exp(-i pi m / 4n) */
+    case FK_EW_HC_POST: {
+        /* bins C_k, k <= n/2 -> halfcomplex r_0..r_{n/2}, i_{(n+1)/2-1}..i_1
+           (doc/reference.texi:917-925), or DHT Re C_k - Im C_k (C_{n-k} =
+           conj C_k gives Re + Im above n/2) */
+        const long long n = a.p0;
+        const bool lo = 2 * j <= n;
+        const cx<R> c = ci[(lo ? j : n - j) * a.is];
+        R v;
+        if (a.p1) v = lo ? c.x - c.y : c.x + c.y;
+        else v = lo ? c.x : c.y;
+        ro[j * a.os] = v;
+        break;
+    }
+    case FK_EW_HC_PRE: {
+        /* halfcomplex -> Hermitian bins k <= n/2 (no imaginary part at DC
+           or, for even n, at n/2) */
+        const long long n = a.p0;
+        const R re = ri[j * a.is];
+        const R im = (j == 0 || 2 * j == n) ? (R)0 : ri[(n - j) * a.is];
+        co[j * a.os] = mk(re, im);
+        break;
+    }
+    case FK_EW_DCT2_PRE: {
+        /* v_j = x_2j (j < ceil(n/2)), v_{n-1-j} = x_{2j+1}; the sine kind
+           negates odd samples (DST-II_k(x) = DCT-II_{n-1-k}((-1)^j x_j)) */
+        const long long n = a.p0, h = (n + 1) / 2;
+        const long long q = j < h ? 2 * j : 2 * (n - 1 - j) + 1;
+        R v = ri[q * a.is];
+        if (a.p1 && (q & 1)) v = -v;
+        ro[j * a.os] = v;
+        break;
+    }
+    case FK_EW_DCT2_POST: {
+        /* D_k = 2 Re(exp(-i pi k / 2n) V_k), V_{n-k} = conj V_k */
+        const long long n = a.p0, k = j;
+        const bool lo = 2 * k <= n;
+        cx<R> V = ci[(lo ? k : n - k) * a.is];
+        if (!lo) V.y = -V.y;
+        const cx<R> w = aux[2 * k];
+        ro[(a.p1 ? n - 1 - k : k) * a.os] = (R)2 * (w.x * V.x - w.y * V.y);
+        break;
+    }
+    case FK_EW_DCT3_PRE: {
+        /* V_k = exp(+i pi k / 2n) (x_k - i x_{n-k}), x_n = 0, k <= n/2; the
+           sine kind reads x reversed (DST-III_k(x) = (-1)^k DCT-III_k(x')) */
+        const long long n = a.p0, k = j;
+        const long long q0 = a.p1 ? n - 1 - k : k, q1 = a.p1 ? k - 1 : n - k;
+        const R xk = ri[q0 * a.is];
+        const R xn = k == 0 ? (R)0 : ri[q1 * a.is];
+        const cx<R> w = aux[2 * k];
+        co[j * a.os] = mk(w.x * xk - w.y * xn, -(w.x * xn + w.y * xk));
+        break;
+    }
+    case FK_EW_DCT3_POST: {
+        /* y_2i = v_i, y_2i+1 = v_{n-1-i} */
+        const long long n = a.p0;
+        const long long q = (j & 1) ? n - 1 - (j >> 1) : (j >> 1);
+        R v = ri[q * a.is];
+        if (a.p1 && (j & 1)) v = -v;
+        ro[j * a.os] = v;
+        break;
+    }
+    case FK_EW_DCT4_PRE: {
+        /* even n: z_m = (x_2m + i x_{n-1-2m}) exp(-i pi (4m+1) / 4n), m < n/2;
+           the sine kind reads x reversed (DST-IV_k(x) = (-1)^k DCT-IV_k(x')) */
+        const long long n = a.p0, m = j;
+        const long long q0 = 2 * m, q1 = n - 1 - 2 * m;
+        const R xa = ri[(a.p1 ? n - 1 - q0 : q0) * a.is];
+        const R xb = ri[(a.p1 ? n - 1 - q1 : q1) * a.is];
+        co[j * a.os] = mul(mk(xa, xb), aux[4 * m + 1]);
+        break;
+    }
+    case FK_EW_DCT4_POST: {
+        /* c_p = Z_p exp(-i pi p / n): Y_2p = 2 Re c_p, Y_{n-1-2p} = -2 Im c_p */
+        const long long n = a.p0;
+        const bool ev = !(j & 1);
+        const long long p = ev ? (j >> 1) : ((n - 1 - j) >> 1);
+        const cx<R> c = mul(ci[p * a.is], aux[4 * p]);
+        R y = ev ? (R)2 * c.x : (R)-2 * c.y;
+        if (a.p1 && !ev) y = -y;
+        ro[j * a.os] = y;
+        break;
+    }
+    case FK_EW_DCT4O_PRE: {
+        /* odd n: a_j = x_j exp(-i pi j / 2n) for j < n, zero up to 2n */
+        const long long n = a.p0;
+        cx<R> v = mk((R)0, (R)0);
+        if (j < n) {
+            const R x = ri[(a.p1 ? n - 1 - j : j) * a.is];
+            const cx<R> w = aux[2 * j];
+            v = mk(x * w.x, x * w.y);
+        }
+        co[j * a.os] = v;
+        break;
+    }
+    case FK_EW_DCT4O_POST: {
+        /* Y_k = 2 Re(exp(-i pi (2k+1) / 4n) A_k), A = 2n-point DFT of a */
+        const cx<R> c = mul(ci[j * a.is], aux[2 * j + 1]);
+        R y = (R)2 * c.x;
+        if (a.p1 && (j & 1)) y = -y;
+        ro[j * a.os] = y;
+        break;
+    }
+    case FK_EW_SYM_PRE: {
+        /* even extension x_0..x_{n-1}, x_{n-2}..x_1 (length 2(n-1)), or odd
+           extension 0, x_0..x_{n-1}, 0, -x_{n-1}..-x_0 (length 2(n+1)) */
+        const long long n = a.p0;
+        R v;
+        if (!a.p1) {
+            v = ri[(j < n ? j : 2 * (n - 1) - j) * a.is];
+        } else if (j == 0 || j == n + 1) {
+            v = (R)0;
+        } else if (j <= n) {
+            v = ri[(j - 1) * a.is];
+        } else {
+            v = -ri[(2 * (n + 1) - j - 1) * a.is];
+        }
+        ro[j * a.os] = v;
+        break;
+    }
+    case FK_EW_SYM_POST:
+        /* DCT-I: Y_k = Re B_k; DST-I: Y_k = -Im B_{k+1} */
+        ro[j * a.os] = a.p1 ? -ci[(j + 1) * a.is].y : ci[j * a.is].x;
+        break;
     default:
         break;
     }

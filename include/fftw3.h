@@ -26,8 +26,12 @@
  * api/execute-split-dft*.c: strides count reals, no sign argument (swap the
  * real and imaginary pointers of each pair for the backward transform).
  *
- * Not provided by this engine (planners return NULL): r2r kinds
- * (plan_*r2r*).
+ * Real-to-real planners (plan_r2r{,_1d,_2d,_3d}, plan_many_r2r,
+ * plan_guru{,64}_r2r, execute_r2r) follow api/plan-many-r2r.c:26-51,
+ * api/plan-guru-r2r.h:23-44 and api/execute-r2r.c:24-28: every kind of
+ * api/fftw3.h:89-100 (R2HC, HC2R, DHT, REDFT00-11, RODFT00-11), any rank,
+ * strides in reals.  A size-1 REDFT00 dim returns NULL
+ * (doc/reference.texi:818-821).  The input is preserved.
  */
 #ifndef FFTW3_H
 #define FFTW3_H

@@ -46,6 +46,9 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [ctypes.c_int, ctypes.c_int, ip, ctypes.c_int, ip, dp, dp]
             f.restype = ctypes.c_int
+        L.orc_r2r.argtypes = [ctypes.c_int, ctypes.c_int, ip, ctypes.c_int, ip,
+                              ctypes.POINTER(ctypes.c_int), dp, dp]
+        L.orc_r2r.restype = ctypes.c_int
         L.orc_dft_direct.argtypes = [ctypes.c_long, ctypes.c_int, dp, dp]
         L.orc_dft_bins.argtypes = [ctypes.c_long, ctypes.c_int, dp,
                                    ctypes.POINTER(ctypes.c_long), ctypes.c_long, dp]
@@ -158,6 +161,33 @@ def irfftn(X, n_last, axes_batch=0, prec=LD):
     out = np.empty(tuple(X.shape[:axes_batch]) + tuple(shape), dtype=np.float64)
     guru_c2r(dims, list(reversed(hdims)), X.reshape(-1), out.reshape(-1), prec)
     return out
+
+
+def guru_r2r(dims, hdims, kinds, x, out, prec=LD):
+    """Real-to-real guru transform; kinds are fftw_r2r_kind values."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    k = (ctypes.c_int * max(1, len(kinds)))(*kinds)
+    rc = lib().orc_r2r(prec, len(dims), _dims(dims), len(hdims), _dims(hdims), k, _dp(x), _dp(out))
+    if rc:
+        raise ValueError("orc_r2r rejected the problem")
+    return out
+
+
+def r2rn(x, kinds, axes_batch=0, prec=LD):
+    """Separable r2r over the trailing len(kinds) axes of a C-contiguous
+    array (leading axes_batch axes are a batch)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    shape = x.shape[axes_batch:]
+    dims, s = [], 1
+    for n in reversed(shape):
+        dims.insert(0, (n, s, s))
+        s *= n
+    hd, hs = [], s
+    for n in reversed(x.shape[:axes_batch]):
+        hd.insert(0, (n, hs, hs))
+        hs *= n
+    out = np.empty_like(x)
+    return guru_r2r(dims, hd, list(kinds), x, out, prec)
 
 
 def dft_direct(x, sign=-1):

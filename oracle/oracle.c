@@ -241,8 +241,150 @@ static void c2r_##SFX(int rank, const orc_iodim *dims, int hrank,            \
     free(buf);                                                                \
 }
 
+/* Real-to-real kinds (doc/reference.texi:913-947 for the kinds, :2021-2350
+   for their definitions).  Each 1-D transform is a complex DFT of the
+   logical symmetric sequence the definition samples: the input is placed
+   at its (half-)sample positions in a zero sequence of length N (2(n-1),
+   2(n+1), 4n or 8n), mirrored with the kind's even / odd symmetry, and the
+   outputs are read from the DFT at the kind's (half-)frequencies.  For
+   example REDFT10 (Y_k = 2 sum x_j cos(pi (j+1/2) k / n)) puts x_j at 2j+1
+   and 4n-2j-1 of a length-4n sequence, whose DFT at k is Y_k.  R2HC, HC2R
+   and DHT are the n-point DFT read as halfcomplex, Hermitian-extended and
+   Re - Im.  Multi-D: the separable product, one dim after another. */
+static long r2r_logical(int kind, long n)
+{
+    switch (kind) {
+    case 3: return 2 * (n - 1);          /* REDFT00 */
+    case 7: return 2 * (n + 1);          /* RODFT00 */
+    case 4: case 5: case 8: case 9: return 4 * n;
+    case 6: case 10: return 8 * n;       /* REDFT11 / RODFT11 */
+    default: return n;                   /* R2HC, HC2R, DHT */
+    }
+}
+
+#define R2R_DRIVER(OR_, SFX)                                                  \
+static void r2r_line_##SFX(int kind, long n, const OR_ *x, OR_ *y,           \
+                           cpx_##SFX *z, cpx_##SFX *f, plan_##SFX *P)         \
+{                                                                             \
+    long N = r2r_logical(kind, n), j, k;                                      \
+    int sign = kind == 1 ? +1 : -1;                                           \
+    for (j = 0; j < N; ++j) { z[j].re = 0; z[j].im = 0; }                     \
+    switch (kind) {                                                           \
+    case 0: case 2:                                                           \
+        for (j = 0; j < n; ++j) z[j].re = x[j];                               \
+        break;                                                                \
+    case 1:                                                                   \
+        z[0].re = x[0];                                                       \
+        for (k = 1; k < n; ++k) {                                             \
+            if (2 * k < n) { z[k].re = x[k]; z[k].im = x[n - k]; }            \
+            else if (2 * k == n) { z[k].re = x[k]; }                          \
+            else { z[k].re = x[n - k]; z[k].im = -x[k]; }                     \
+        }                                                                     \
+        break;                                                                \
+    case 3:                                                                   \
+        for (j = 0; j < n; ++j) { z[j].re += x[j]; if (j && j < n - 1) z[N - j].re += x[j]; } \
+        break;                                                                \
+    case 7:                                                                   \
+        for (j = 0; j < n; ++j) { z[j + 1].re = x[j]; z[N - j - 1].re = -x[j]; } \
+        break;                                                                \
+    case 5: case 6: case 9: case 10: {                                        \
+        OR_ s = (kind == 9 || kind == 10) ? -1 : 1;                           \
+        for (j = 0; j < n; ++j) { z[2 * j + 1].re = x[j]; z[N - 2 * j - 1].re = s * x[j]; } \
+        break;                                                                \
+    }                                                                         \
+    case 4:                                                                   \
+        z[0].re = x[0];                                                       \
+        for (j = 1; j < n; ++j) { z[j].re = x[j]; z[N - j].re = x[j]; }       \
+        break;                                                                \
+    case 8:                                                                   \
+        for (j = 0; j < n; ++j) {                                             \
+            OR_ v = j == n - 1 ? x[j] / 2 : x[j];                             \
+            z[j + 1].re += v;                                                 \
+            z[N - j - 1].re -= v;                                             \
+        }                                                                     \
+        break;                                                                \
+    }                                                                         \
+    fft_##SFX(P, sign, z, 1, f);                                              \
+    for (k = 0; k < n; ++k) {                                                 \
+        switch (kind) {                                                       \
+        case 0: y[k] = 2 * k <= n ? f[k].re : f[n - k].im; break;             \
+        case 1: y[k] = f[k].re; break;                                        \
+        case 2: y[k] = f[k].re - f[k].im; break;                              \
+        case 3: case 5: y[k] = f[k].re; break;                                \
+        case 4: case 6: y[k] = f[2 * k + 1].re; break;                        \
+        case 7: case 9: y[k] = -f[k + 1].im; break;                           \
+        case 8: case 10: y[k] = -f[2 * k + 1].im; break;                      \
+        }                                                                     \
+    }                                                                         \
+}                                                                             \
+                                                                              \
+static void r2r_##SFX(int rank, const orc_iodim *dims, int hrank,            \
+                      const orc_iodim *hdims, const int *kinds,              \
+                      const double *in, double *out)                          \
+{                                                                             \
+    long total = prod_n(rank, dims), H = prod_n(hrank, hdims), i;           \
+    int d;                                                                    \
+    OR_ *buf;                                                                 \
+    if (H == 0) return;                                                       \
+    buf = (OR_ *)malloc(sizeof(OR_) * H * total);                            \
+    for (i = 0; i < H * total; ++i) {                                         \
+        long h = i / total, e = i % total;                                    \
+        buf[i] = (OR_)in[off_of(h, hrank, hdims, 0) + off_of(e, rank, dims, 0)]; \
+    }                                                                         \
+    for (d = rank - 1; d >= 0; --d) {                                         \
+        long nd = dims[d].n, inner = 1, lines, L, N = r2r_logical(kinds[d], nd); \
+        int e;                                                                \
+        plan_##SFX *P;                                                        \
+        prepare_##SFX(N);                                                     \
+        P = get_plan_##SFX(N);                                                \
+        for (e = d + 1; e < rank; ++e) inner *= dims[e].n;                    \
+        lines = H * (total / nd);                                             \
+        _Pragma("omp parallel")                                               \
+        {                                                                     \
+            OR_ *tx = (OR_ *)malloc(sizeof(OR_) * nd);                        \
+            OR_ *ty = (OR_ *)malloc(sizeof(OR_) * nd);                        \
+            cpx_##SFX *z = (cpx_##SFX *)malloc(sizeof(cpx_##SFX) * N);        \
+            cpx_##SFX *f = (cpx_##SFX *)malloc(sizeof(cpx_##SFX) * N);        \
+            _Pragma("omp for schedule(static)")                               \
+            for (L = 0; L < lines; ++L) {                                     \
+                long h = L / (total / nd), r = L % (total / nd);              \
+                long outer = r / inner, in_ = r % inner, j;                   \
+                OR_ *base = buf + h * total + outer * nd * inner + in_;       \
+                for (j = 0; j < nd; ++j) tx[j] = base[j * inner];             \
+                r2r_line_##SFX(kinds[d], nd, tx, ty, z, f, P);                \
+                for (j = 0; j < nd; ++j) base[j * inner] = ty[j];             \
+            }                                                                 \
+            free(tx); free(ty); free(z); free(f);                             \
+        }                                                                     \
+    }                                                                         \
+    for (i = 0; i < H * total; ++i) {                                         \
+        long h = i / total, e = i % total;                                    \
+        out[off_of(h, hrank, hdims, 1) + off_of(e, rank, dims, 1)] = (double)buf[i]; \
+    }                                                                         \
+    free(buf);                                                                \
+}
+
 DRIVERS(long double, ld)
 DRIVERS(double, d)
+R2R_DRIVER(long double, ld)
+R2R_DRIVER(double, d)
+
+int orc_r2r(int prec, int rank, const orc_iodim *dims, int hrank,
+            const orc_iodim *hdims, const int *kinds, const double *in,
+            double *out)
+{
+    int i;
+    if (!dims_ok(rank, dims, hrank, hdims)) return -1;
+    for (i = 0; i < rank; ++i) {
+        if (kinds[i] < 0 || kinds[i] > 10) return -1;
+        if (kinds[i] == 3 && dims[i].n == 1) return -1; /* REDFT00 of size 1 */
+    }
+    if (prec == ORC_LD)
+        r2r_ld(rank, dims, hrank, hdims, kinds, in, out);
+    else
+        r2r_d(rank, dims, hrank, hdims, kinds, in, out);
+    return 0;
+}
 
 int orc_dft(int prec, int rank, const orc_iodim *dims, int hrank,
             const orc_iodim *hdims, const double *in, double *out, int sign)

@@ -19,7 +19,11 @@
  *   - twiddles exp(2*pi*i*m/n) computed with exact integer reduction of m
  *     and octant symmetry before cos/sin (kernel/trig.c:57-80);
  *   - Bluestein chirp w_m = exp(i*pi*m^2/n) with m^2 reduced mod 2n
- *     (dft/bluestein.c:37-50).
+ *     (dft/bluestein.c:37-50);
+ *   - r2r kinds: halfcomplex R2HC / HC2R, DHT, DCT-I..IV (REDFTab) and
+ *     DST-I..IV (RODFTab) as defined in doc/reference.texi:913-947 and
+ *     :2021-2350, each computed as a complex DFT of its logical symmetric
+ *     sequence (length 2(n-1), 2(n+1), 4n or 8n).
  *
  * Internal precision: ORC_LD computes in x87 long double (64-bit mantissa),
  * the accuracy oracle; ORC_D computes the same algorithm in double, which is
@@ -54,6 +58,14 @@ int orc_r2c(int prec, int rank, const orc_iodim *dims, int hrank,
    out: reals.  The input is not modified. */
 int orc_c2r(int prec, int rank, const orc_iodim *dims, int hrank,
             const orc_iodim *hdims, const double *in, double *out);
+
+/* Real-to-real transforms, guru form, kinds[d] = fftw_r2r_kind value of
+   dim d (0 R2HC, 1 HC2R, 2 DHT, 3-6 REDFT00/01/10/11, 7-10 RODFT00/01/10/11).
+   Reals in and out; the input is not modified.  -1 on bad arguments,
+   including a size-1 REDFT00 dim. */
+int orc_r2r(int prec, int rank, const orc_iodim *dims, int hrank,
+            const orc_iodim *hdims, const int *kinds, const double *in,
+            double *out);
 
 /* Brute-force O(n^2) DFT in long double with exact (j*k mod n) reduction.
    Contiguous interleaved complex in/out (out != in). */

@@ -1,0 +1,181 @@
+"""Real-to-real kinds on the GPU (plan_many_r2r / plan_guru{,64}_r2r /
+plan_r2r_{1,2,3}d, execute_r2r) against the oracle on the same inputs.
+
+Covers every kind of api/fftw3.h:89-100 at small, odd, prime, power-of-two
+and large sizes, both precisions, in-place and out-of-place, batches,
+multi-dimensional mixed kinds, guru strides (transposed and padded), new
+arrays at execute, the size-1 and size-2 special cases, input
+preservation, the NULL cases (size-1 REDFT00, unknown kind) and wisdom.
+Bars: relative L2 of TOL (1e-13 FP64, 2e-6 FP32) against the long-double
+oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import fftw3_amd as fa
+import oracle
+from gpu_util import TOL, inputs_r, rdt
+
+pytestmark = pytest.mark.gpu
+
+ALL = list(range(11))
+NAMES = ["r2hc", "hc2r", "dht", "e00", "e01", "e10", "e11", "o00", "o01", "o10", "o11"]
+SIZES = [1, 2, 3, 4, 5, 8, 15, 16, 17, 64, 100, 127, 128, 1000, 1024, 4093, 4096, 65536]
+
+
+def many(prec, kind, n, howmany, seed, inplace=False, flags=fa.FFTW_ESTIMATE):
+    x = inputs_r(n * howmany, seed, prec)
+    a = fa.DeviceBuffer.from_array(x)
+    o = a if inplace else fa.DeviceBuffer(x.nbytes)
+    p = fa.plan_many_r2r(prec, [n], howmany, a.ptr, None, 1, n, o.ptr, None, 1, n, [kind], flags)
+    if not (flags & fa.FFTW_ESTIMATE):
+        a.upload(x)
+    p.execute()
+    y = o.download(rdt(prec))
+    xin = None if inplace else a.download(rdt(prec))
+    p.destroy()
+    return x, y, xin
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kind", ALL, ids=NAMES)
+def test_r2r_1d_sizes(prec, kind):
+    for n in SIZES:
+        if kind == 3 and n == 1:
+            continue
+        h = 3 if n <= 4096 else 2
+        print("r2r", prec, NAMES[kind], n, flush=True)
+        x, y, xin = many(prec, kind, n, h, 100 + n)
+        ref = oracle.r2rn(x.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
+        assert oracle.rel_l2(y, ref) < TOL[prec], (NAMES[kind], n)
+        assert np.array_equal(xin, x), "input not preserved"
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kind", ALL, ids=NAMES)
+def test_r2r_inplace(prec, kind):
+    for n in (6, 7, 256, 1001):
+        x, y, _ = many(prec, kind, n, 5, 7 + n, inplace=True)
+        ref = oracle.r2rn(x.astype(np.float64).reshape(5, n), [kind], axes_batch=1).reshape(-1)
+        assert oracle.rel_l2(y, ref) < TOL[prec], (NAMES[kind], n)
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kinds,shape", [((5, 5), (64, 48)),
+                                         ((4, 10), (33, 20)),
+                                         ((0, 2, 7), (6, 10, 12)),
+                                         ((3, 1, 6), (9, 8, 5)),
+                                         ((6, 8), (1, 40)),
+                                         ((5, 5, 5), (32, 32, 32))])
+@pytest.mark.parametrize("inplace", [False, True])
+def test_r2r_multid(prec, kinds, shape, inplace):
+    N = int(np.prod(shape))
+    x = inputs_r(2 * N, 21, prec)
+    a = fa.DeviceBuffer.from_array(x)
+    o = a if inplace else fa.DeviceBuffer(x.nbytes)
+    p = fa.plan_many_r2r(prec, list(shape), 2, a.ptr, None, 1, N, o.ptr, None, 1, N, list(kinds))
+    p.execute()
+    y = o.download(rdt(prec))
+    ref = oracle.r2rn(x.astype(np.float64).reshape((2,) + shape), kinds, axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(y, ref) < TOL[prec]
+    p.destroy()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("wide", [False, True])
+def test_r2r_guru_strides(prec, wide):
+    """Transposed input, padded output rows, two howmany dims."""
+    n0, n1, h0, h1 = 12, 20, 3, 2
+    kinds = [5, 9]
+    span_in = n0 * n1 * h0 * h1
+    pad = n1 + 5
+    span_out = n0 * pad * h0 * h1
+    x = inputs_r(span_in, 31, prec)
+    # input: element (i0, i1, b0, b1) at i1*n0 + i0 + b0*(n0*n1) + b1*(n0*n1*h0)
+    dims = [(n0, 1, pad), (n1, n0, 1)]
+    hdims = [(h0, n0 * n1, n0 * pad), (h1, n0 * n1 * h0, n0 * pad * h0)]
+    a = fa.DeviceBuffer.from_array(x)
+    o = fa.DeviceBuffer(span_out * x.itemsize)
+    o.upload(np.full(span_out, -7, rdt(prec)))
+    p = fa.plan_guru_r2r(prec, dims, hdims, a.ptr, o.ptr, kinds, wide=wide)
+    p.execute()
+    y = o.download(rdt(prec))
+    out = np.full(span_out, -7.0)
+    oracle.guru_r2r(dims, hdims, kinds, x.astype(np.float64), out)
+    assert oracle.rel_l2(y, out) < TOL[prec]
+    # padding untouched
+    assert np.all(y.reshape(h1, h0, n0, pad)[..., n1:] == -7)
+    p.destroy()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_r2r_execute_new_arrays_and_wrappers(prec):
+    L = fa.lib()
+    P = fa.prefix(prec)
+    n0, n1, n2 = 8, 6, 10
+    x = inputs_r(n0 * n1 * n2, 41, prec)
+    a, b = fa.DeviceBuffer.from_array(x), fa.DeviceBuffer(x.nbytes)
+    a2, b2 = fa.DeviceBuffer.from_array(x[::-1].copy()), fa.DeviceBuffer(x.nbytes)
+    h1 = getattr(L, P + "plan_r2r_1d")(n0 * n1 * n2, a.ptr, b.ptr, 6, fa.FFTW_ESTIMATE)
+    h2 = getattr(L, P + "plan_r2r_2d")(n0 * n1, n2, a.ptr, b.ptr, 4, 2, fa.FFTW_ESTIMATE)
+    h3 = getattr(L, P + "plan_r2r_3d")(n0, n1, n2, a.ptr, b.ptr, 5, 8, 0, fa.FFTW_ESTIMATE)
+    cases = [(fa.Plan(h1, prec), [6], (n0 * n1 * n2,)), (fa.Plan(h2, prec), [4, 2], (n0 * n1, n2)),
+             (fa.Plan(h3, prec), [5, 8, 0], (n0, n1, n2))]
+    for p, kinds, shape in cases:
+        p.execute()
+        ref = oracle.r2rn(x.astype(np.float64).reshape(shape), kinds)
+        assert oracle.rel_l2(b.download(rdt(prec)), ref.reshape(-1)) < TOL[prec]
+        p.execute_r2r(a2.ptr, b2.ptr)
+        ref2 = oracle.r2rn(x[::-1].astype(np.float64).reshape(shape), kinds)
+        assert oracle.rel_l2(b2.download(rdt(prec)), ref2.reshape(-1)) < TOL[prec]
+        p.destroy()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_r2r_null_cases(prec):
+    b = fa.DeviceBuffer(1 << 12)
+    with pytest.raises(RuntimeError):
+        fa.plan_many_r2r(prec, [1], 1, b.ptr, None, 1, 1, b.ptr, None, 1, 1, [3])
+    with pytest.raises(RuntimeError):
+        fa.plan_many_r2r(prec, [8], 1, b.ptr, None, 1, 8, b.ptr, None, 1, 8, [11])
+    # size-1 REDFT00 inside a multi-D problem is also undefined
+    with pytest.raises(RuntimeError):
+        fa.plan_many_r2r(prec, [4, 1], 1, b.ptr, None, 1, 4, b.ptr, None, 1, 4, [5, 3])
+    # a rank-0 / all-trivial problem is a copy
+    x = inputs_r(16, 3, prec)
+    a, o = fa.DeviceBuffer.from_array(x), fa.DeviceBuffer(x.nbytes)
+    p = fa.plan_many_r2r(prec, [1], 16, a.ptr, None, 1, 1, o.ptr, None, 1, 1, [0])
+    p.execute()
+    assert np.array_equal(o.download(rdt(prec)), x)
+    p.destroy()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kind", [5, 4, 6, 10, 0])
+def test_r2r_measure_and_wisdom(prec, kind):
+    L = fa.lib()
+    P = fa.prefix(prec)
+    getattr(L, P + "forget_wisdom")()
+    n, h = 4096, 64
+    x, y, _ = many(prec, kind, n, h, 55, flags=fa.FFTW_MEASURE)
+    ref = oracle.r2rn(x.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(y, ref) < TOL[prec]
+    # the timed plan is now wisdom: WISDOM_ONLY planning succeeds
+    x2, y2, _ = many(prec, kind, n, h, 56, flags=fa.FFTW_ESTIMATE | fa.FFTW_WISDOM_ONLY)
+    ref2 = oracle.r2rn(x2.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(y2, ref2) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec,shape,kinds", [("float", (1024, 1024), (5, 5)),
+                                              ("double", (512, 512), (4, 4)),
+                                              ("float", (1 << 20,), (6,))])
+def test_r2r_large(prec, shape, kinds):
+    N = int(np.prod(shape))
+    x = inputs_r(N, 61, prec)
+    a, o = fa.DeviceBuffer.from_array(x), fa.DeviceBuffer(x.nbytes)
+    p = fa.plan_many_r2r(prec, list(shape), 1, a.ptr, None, 1, N, o.ptr, None, 1, N, list(kinds))
+    p.execute()
+    ref = oracle.r2rn(x.astype(np.float64).reshape(shape), kinds).reshape(-1)
+    assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
+    p.destroy()

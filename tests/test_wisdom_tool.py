@@ -1,8 +1,8 @@
 """fftw-wisdom pre-planning tool (the reference's tools/fftw-wisdom.c).
 
 CPU part: the problem-string grammar of libbench2/problem.c:229-318 as the
-tool accepts it (type, placement, direction, split prefix, geometry,
-*howmany), the reference's canonical set (tools/fftw-wisdom.c:134-147 and
+tool accepts it (type, placement, direction, split prefix, geometry, r2r
+kinds after each dim, *howmany), the reference's canonical set (tools/fftw-wisdom.c:134-147 and
 :291-310: 8 types per 1-D size, in-place types only for multi-dimensional
 sizes, 300 problems), ordering by size, and rejection of what it cannot
 plan.  GPU part: the wisdom it writes makes FFTW_WISDOM_ONLY planning
@@ -40,7 +40,22 @@ def test_problem_grammar():
 
 
 @need_tool
-@pytest.mark.parametrize("bad", ["cof12q", "cof", "c0", "cofx4", "/rof64", "kf64", "cof8*"])
+@need_tool
+def test_r2r_grammar():
+    # libbench2/problem.c:152-195: a kind letter after each dim, R2HC default
+    r = run("-l", "ko1024e10", "ki64x32o11*3", "k8", "ki16hx8b", "k5o00")
+    assert r.returncode == 0, r.stderr
+    d = {l.split()[0]: l for l in r.stdout.strip().splitlines()}
+    assert d["ko1024e10"] == "ko1024e10 r2r out-of-place rank=1 n=1024 kind=redft10 howmany=1"
+    assert d["ki64x32o11*3"] == "ki64x32o11*3 r2r in-place rank=2 n=64x32 kind=r2hc,rodft11 howmany=3"
+    assert d["k8"].endswith("kind=r2hc howmany=1")
+    assert "kind=dht,hc2r" in d["ki16hx8b"]
+    assert "kind=rodft00" in d["k5o00"]
+
+
+@need_tool
+@pytest.mark.parametrize("bad", ["cof12q", "cof", "c0", "cofx4", "/rof64", "k64e2", "k64o111", "/k64",
+                                 "cof8*", "rof64e10"])
 def test_rejects(bad):
     r = run("-l", bad)
     assert r.returncode != 0 and "cannot parse" in r.stderr
@@ -78,6 +93,7 @@ o = fa.DeviceBuffer(1 << 22)
 fl = fa.FFTW_ESTIMATE | fa.FFTW_WISDOM_ONLY
 p = fa.plan_many_dft(prec, [1024], 1, b.ptr, None, 1, 1024, o.ptr, None, 1, 1024, -1, fl)
 q = fa.plan_many_dft_r2c(prec, [64, 64], 1, b.ptr, None, 1, 64 * 66, b.ptr, None, 1, 64 * 33, fl)
+k = fa.plan_many_r2r(prec, [512], 4, b.ptr, None, 1, 512, o.ptr, None, 1, 512, [fa.FFTW_REDFT10], fl)
 try:
     fa.plan_many_dft(prec, [2048], 1, b.ptr, None, 1, 2048, o.ptr, None, 1, 2048, -1, fl)
     raise SystemExit("no wisdom for 2048 expected")
@@ -92,13 +108,13 @@ print("ok")
 @pytest.mark.parametrize("prec", ["double", "float"])
 def test_wisdom_only_after_tool(tmp_path, prec):
     out = str(tmp_path / "w.txt")
-    args = ["-m", "-n", "-o", out, "cof1024", "rif64x64"]
+    args = ["-m", "-n", "-o", out, "cof1024", "rif64x64", "ko512e10*4"]
     if prec == "float":
         args.insert(0, "-s")
     r = run(*args, timeout=120)
     assert r.returncode == 0, r.stderr
     txt = open(out).read()
-    assert txt.startswith("(fftw3-amd-gfx950-wisdom #x") and txt.count("(#x") == 2
+    assert txt.startswith("(fftw3-amd-gfx950-wisdom #x") and txt.count("(#x") == 3
     r = subprocess.run([sys.executable, "-c", CHECK % {"root": ROOT, "path": out, "prec": prec}],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
