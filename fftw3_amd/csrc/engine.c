@@ -446,11 +446,22 @@ static void fill_batch3(long long *nbv, long long *ids, long long *ods, long lon
     *nl = t;
 }
 
-static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b, size_t isz,
+static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b0, size_t isz,
                        size_t osz)
 {
     long long idx[FK_MAXRANK * 2] = {0};
-    int nloop = nb > 3 ? nb - 3 : 0, i;
+    int nloop = nb > 3 ? nb - 3 : 0, i, j;
+    fk_dim b[FK_MAXRANK * 2];
+    memcpy(b, b0, sizeof(fk_dim) * nb);
+    /* beyond three batch dims, keep the two largest after b[0] (whose place
+       the thread mapping may depend on) in the kernel and loop over the
+       smallest: fewer, larger launches */
+    for (i = 2; nb > 3 && i < nb; ++i)
+        for (j = i; j > 1 && b[j].n > b[j - 1].n; --j) {
+            fk_dim t = b[j];
+            b[j] = b[j - 1];
+            b[j - 1] = t;
+        }
     for (;;) {
         long long ioff = proto->ioff, ooff = proto->ooff;
         long long i2off = proto->i2off, o2off = proto->o2off;
@@ -1379,6 +1390,48 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
         emit_loops(c, &proto, nb, b, rs, rs);
         return c->err ? -1 : 0;
     }
+    {
+        /* one pass: the kind's reordering, extension and twiddles fused
+           into a real kernel of half length L (R2C for R2HC / DHT /
+           REDFT10 / RODFT10 with L = n/2, REDFT00 with L = n-1, RODFT00
+           with L = n+1; C2R for HC2R / REDFT01 / RODFT01 with L = n/2) */
+        static const int rmode[] = {FK_RM_HC, FK_RM_HC2R, FK_RM_DHT, FK_RM_E00, FK_RM_DCT3,
+                                    FK_RM_DCT2, 0, FK_RM_O00, FK_RM_DST3, FK_RM_DST2, 0};
+        const int rm = rmode[kind];
+        const int io = (rm == FK_RM_HC2R || rm == FK_RM_DCT3 || rm == FK_RM_DST3) ? FK_IO_C2R_RR
+                                                                                : FK_IO_R2C_RR;
+        const long long L = rm == FK_RM_E00 ? n - 1 : rm == FK_RM_O00 ? n + 1 : n / 2;
+        const int ok = rm && (rm == FK_RM_E00 || rm == FK_RM_O00 || n % 2 == 0);
+        const fk_kernel *k = ok ? pick_variant(c, prec, L, FK_ROW, io) : 0;
+        if (k) {
+            size_t before = (size_t)P->npass;
+            unsigned fl = 0;
+            int unit = (rm == FK_RM_HC || rm == FK_RM_DHT || rm == FK_RM_HC2R) &&
+                       (io == FK_IO_R2C_RR ? is : os) == 1 &&
+                       ((uintptr_t)(io == FK_IO_R2C_RR ? iptr : optr) % (2 * rs)) == 0;
+            const void *wk = line_table(prec, 2 * L, L + 1, c->st);
+            for (i = 0; i < nb; ++i) unit &= ((io == FK_IO_R2C_RR ? b[i].is : b[i].os) % 2) == 0;
+            const int twid = rm == FK_RM_DCT2 || rm == FK_RM_DST2 || rm == FK_RM_DCT3 ||
+                             rm == FK_RM_DST3;
+            if (twid) wt = line_table(prec, 8 * n, 2 * n, c->st);
+            if (!wk || (twid && !wt)) {
+                c->err = 1;
+                set_err("twiddle table allocation failed");
+                return -1;
+            }
+            if (unit) fl |= FK_REAL_UNIT;
+            emit_stockham(c, k, is, os, nb, b, ib, 0, ob, 0, fl, rs, rs);
+            for (; before < (size_t)P->npass; ++before) {
+                fk_pass *p = P->pass + before;
+                p->a.aux = wk;
+                p->a.rtw = wt;
+                p->a.rmode = (unsigned)rm;
+                p->bytes = 2.0 * (double)batch_count(nb, b) * (double)n * rs;
+            }
+            if (unit) P->align_need = (unsigned)(2 * rs);
+            return c->err ? -1 : 0;
+        }
+    }
     switch (kind) {
     case FK_K_R2HC:
     case FK_K_DHT:
@@ -2067,7 +2120,7 @@ int fk_plan_use_mfma(fk_plan *p)
         const fk_kernel *k = ps->k;
         int all16 = 1;
         if (ps->type != FK_PASS_STOCKHAM || k->io == FK_IO_R2C || k->io == FK_IO_C2R ||
-            k->io == FK_IO_BLUE)
+            k->io == FK_IO_BLUE || k->io == FK_IO_R2C_RR || k->io == FK_IO_C2R_RR)
             continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (!all16 || (k->L != 16 && k->L != 256 && k->L != 4096)) continue;
@@ -2168,8 +2221,8 @@ double fk_plan_bytes(const fk_plan *p)
 
 static const char *io_name(int io)
 {
-    static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein"};
-    return io >= 0 && io < 5 ? n[io] : "?";
+    static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r"};
+    return io >= 0 && io < 7 ? n[io] : "?";
 }
 
 int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)

@@ -20,7 +20,9 @@ enum {
     FK_IO_TOUT = 1,   /* store through LDS with lines-fastest mapping */
     FK_IO_R2C = 2,    /* real input viewed as L complex; store n/2+1 bins */
     FK_IO_C2R = 3,    /* load n/2+1 bins, half-length inverse, real output */
-    FK_IO_BLUE = 4    /* fused Bluestein convolution of length L */
+    FK_IO_BLUE = 4,   /* fused Bluestein convolution of length L */
+    FK_IO_R2C_RR = 5, /* FK_IO_R2C with a fused real-to-real kind (rmode) */
+    FK_IO_C2R_RR = 6  /* FK_IO_C2R with a fused real-to-real kind (rmode) */
 };
 
 /* flags */
@@ -30,6 +32,20 @@ enum {
                              i0 by W_N^(k*i0) (four-step twiddle) */
 #define FK_SCALE     8u   /* multiply output by scale */
 #define FK_REAL_UNIT 16u  /* R2C: real input stride is 1 / C2R: output stride 1 */
+
+/* real-to-real kinds fused into the FK_IO_R2C_RR / FK_IO_C2R_RR kernels (rmode):
+   the real line is read or written in the kind's order, with the kind's
+   twiddle exp(-i pi m / 4n) from rtw */
+enum { FK_RM_NONE = 0,
+       FK_RM_HC = 1,      /* R2C: halfcomplex output (R2HC) */
+       FK_RM_DHT = 2,     /* R2C: Hartley output */
+       FK_RM_DCT2 = 3,    /* R2C: REDFT10 */
+       FK_RM_DST2 = 4,    /* R2C: RODFT10 */
+       FK_RM_HC2R = 5,    /* C2R: halfcomplex input (HC2R) */
+       FK_RM_DCT3 = 6,    /* C2R: REDFT01 */
+       FK_RM_DST3 = 7,    /* C2R: RODFT01 */
+       FK_RM_E00 = 8,     /* R2C of the even extension (length 2L = 2(n-1)): REDFT00 */
+       FK_RM_O00 = 9 };   /* R2C of the odd extension (length 2L = 2(n+1)): RODFT00 */
 
 typedef struct {
     const void *in;
@@ -50,6 +66,9 @@ typedef struct {
     double scale;
     unsigned flags;
     unsigned xcd_remap;       /* nonzero: spread tiles XCD-contiguously */
+    unsigned rmode;           /* FK_RM_*: fused real-to-real kind */
+    unsigned pad_;
+    const void *rtw;          /* FK_RM_DCT2/3, FK_RM_DST2/3: exp(-i pi m / 4n), m < 2n */
 } fk_args;
 
 typedef struct {

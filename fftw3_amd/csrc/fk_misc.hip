@@ -141,17 +141,43 @@ __global__ void k_fill_gauss(R *out, long long count, unsigned long long seed,
     }
 }
 
+/* Thread mapping of the elementwise passes.  Lines of n > 128 elements:
+   each workgroup covers a segment of EW_BS elements of one line (spl
+   segments per line), so the line and its batch offsets are the same for
+   the whole workgroup.  Shorter lines: lpb whole lines per workgroup.
+   Blocks start at line line0. */
+constexpr int EW_BS = 256;
+
 template <typename R>
-__global__ void k_ew(fk_ew_args a)
+__global__ void __launch_bounds__(EW_BS) k_ew(fk_ew_args a, int lpb, long long spl, long long line0)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = a.nlines * a.n;
-    if (t >= total) return;
-    const long long j = t % a.n;
-    long long q = t / a.n;
-    const long long i0 = q % a.nb[0];
-    q /= a.nb[0];
-    const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
+    long long line, jj;
+    if (lpb > 1) {
+        const unsigned nn = (unsigned)a.n, li = threadIdx.x / nn;
+        if (li >= (unsigned)lpb) return;
+        line = line0 + (long long)blockIdx.x * lpb + li;
+        jj = (long long)(threadIdx.x - li * nn);
+    } else {
+        const long long b = blockIdx.x, ls = b / spl;
+        line = line0 + ls;
+        jj = (b - ls * spl) * EW_BS + threadIdx.x;
+        if (jj >= a.n) return;
+    }
+    if (line >= a.nlines) return;
+    const long long j = jj;
+    long long i0, i1, i2;
+    if (a.nlines <= 0xffffffffLL) {
+        const unsigned l = (unsigned)line, n0 = (unsigned)a.nb[0], n1 = (unsigned)a.nb[1];
+        const unsigned q = l / n0;
+        i0 = l - q * n0;
+        i1 = q % n1;
+        i2 = q / n1;
+    } else {
+        const long long q = line / a.nb[0];
+        i0 = line - q * a.nb[0];
+        i1 = q % a.nb[1];
+        i2 = q / a.nb[1];
+    }
     const long long bi = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
     const long long bo = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
     const bool swi = a.flags & FK_SWAP_IN, swo = a.flags & FK_SWAP_OUT;
@@ -482,12 +508,22 @@ int fkm_fill(void *out, int prec, long long count, unsigned long long seed,
 
 int fkm_ew(int prec, const fk_ew_args *a, hipStream_t st)
 {
-    long long total = a->nlines * a->n;
-    if (total <= 0) return 0;
-    if (prec == 0)
-        hipLaunchKernelGGL(k_ew<double>, dim3(nblocks(total, 256)), dim3(256), 0, st, *a);
-    else
-        hipLaunchKernelGGL(k_ew<float>, dim3(nblocks(total, 256)), dim3(256), 0, st, *a);
+    if (a->nlines <= 0 || a->n <= 0) return 0;
+    const int lpb = a->n <= EW_BS / 2 ? (int)(EW_BS / a->n) : 1;
+    const long long spl = lpb > 1 ? 1 : (a->n + EW_BS - 1) / EW_BS;
+    const long long total = lpb > 1 ? (a->nlines + lpb - 1) / lpb : a->nlines * spl;
+    long long chunk = ((1LL << 30) / spl) * spl;        /* whole lines per launch */
+    if (chunk < spl) chunk = spl;
+    for (long long b0 = 0; b0 < total; b0 += chunk) {
+        const long long nb = total - b0 < chunk ? total - b0 : chunk;
+        const long long line0 = lpb > 1 ? b0 * lpb : b0 / spl;
+        if (prec == 0)
+            hipLaunchKernelGGL(k_ew<double>, dim3((unsigned)nb), dim3(EW_BS), 0, st, *a, lpb, spl,
+                               line0);
+        else
+            hipLaunchKernelGGL(k_ew<float>, dim3((unsigned)nb), dim3(EW_BS), 0, st, *a, lpb, spl,
+                               line0);
+    }
     return (int)hipGetLastError();
 }
 

@@ -258,16 +258,40 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             }
         }
-    } else if constexpr (IO == FK_IO_R2C) {
+    } else if constexpr (IO == FK_IO_R2C || IO == FK_IO_R2C_RR) {
         /* real line of 2L samples viewed as L complex z_j = x_2j + i x_2j+1;
            after the L-point DFT Z, X[k] = E[k] + W_2L^k O[k] with
            E = (Z[k] + conj Z[L-k])/2, O = (Z[k] - conj Z[L-k])/(2i),
            k = 0..L (the role of rdft/ct-hc2c.c:197-210's HC2C_VIA_DFT). */
         const R *inr = reinterpret_cast<const R *>(a.in) + bin;
         cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+        R *outr = reinterpret_cast<R *>(a.out) + bout;
         const cx<R> *wk = reinterpret_cast<const cx<R> *>(a.aux);
+        const cx<R> *rt = reinterpret_cast<const cx<R> *>(a.rtw);
+        /* fused real-to-real kind; a compile-time FK_RM_NONE in the plain
+           r2c / c2r instantiations */
+        const unsigned rm = (IO == FK_IO_R2C_RR || IO == FK_IO_C2R_RR) ? a.rmode : 0u;
+        constexpr long long N2 = 2LL * L;
+        /* REDFT10 / RODFT10 read the line reordered (Makhoul):
+           v_m = x_2m for m < L, x_{2(n-1-m)+1} above; the sine kind
+           negates the odd samples (RODFT10_k(x) = REDFT10_{n-1-k}((-1)^j x_j)) */
+        auto vld = [&](int m) -> R {
+            const long long q = m < L ? 2LL * m : 2 * (N2 - 1 - m) + 1;
+            const R v = inr[q * a.is];
+            return (rm == FK_RM_DST2 && m >= L) ? -v : v;
+        };
+        /* REDFT00 / RODFT00 read the even extension x_0..x_{n-1},
+           x_{n-2}..x_1 (n = L+1) or the odd one 0, x_0..x_{n-1}, 0,
+           -x_{n-1}..-x_0 (n = L-1) of the line */
+        auto eld = [&](long long m) -> R {
+            if (rm == FK_RM_E00) return inr[(m <= L ? m : N2 - m) * a.is];
+            if (m == 0 || m == L) return (R)0;
+            return m < L ? inr[(m - 1) * a.is] : -inr[(N2 - m - 1) * a.is];
+        };
         auto gld = [&](int j) -> cx<R> {
             if (!act) return mk((R)0, (R)0);
+            if (rm == FK_RM_DCT2 || rm == FK_RM_DST2) return mk(vld(2 * j), vld(2 * j + 1));
+            if (rm == FK_RM_E00 || rm == FK_RM_O00) return mk(eld(2 * j), eld(2 * j + 1));
             if (fl & FK_REAL_UNIT) return reinterpret_cast<const cx<R> *>(inr)[j];
             return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
         };
@@ -297,18 +321,67 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
                     cx<R> X = mk(er + orr * w[i].x - oi * w[i].y, ei + orr * w[i].y + oi * w[i].x);
                     if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
-                    if (act) out[bout + (long long)k * a.os] = X;
+                    if (act && rm == FK_RM_NONE) {
+                        out[bout + (long long)k * a.os] = X;
+                    } else if (act && rm == FK_RM_E00) {
+                        outr[(long long)k * a.os] = X.x;             /* DCT-I: Re X_k */
+                    } else if (act && rm == FK_RM_O00) {
+                        if (k >= 1 && k < L) outr[(long long)(k - 1) * a.os] = -X.y; /* DST-I */
+                    } else if (act) {
+                        /* bin k gives two real outputs, k and n-k:
+                           R2HC r_k / i_k; DHT Re -+ Im; REDFT10
+                           c = exp(-i pi k / 2n) X: 2 Re c, -2 Im c */
+                        R lo, hi;
+                        if (rm == FK_RM_HC) {
+                            lo = X.x;
+                            hi = X.y;
+                        } else if (rm == FK_RM_DHT) {
+                            lo = X.x - X.y;
+                            hi = X.x + X.y;
+                        } else {
+                            const cx<R> c = mul(X, rt[2 * k]);
+                            lo = (R)2 * c.x;
+                            hi = (R)-2 * c.y;
+                        }
+                        long long ka = k, kb = N2 - k;
+                        if (rm == FK_RM_DST2) {
+                            ka = N2 - 1 - k;
+                            kb = k - 1;
+                        }
+                        outr[ka * a.os] = lo;
+                        if (k != 0 && k != L) outr[kb * a.os] = hi;
+                    }
                 }
             });
         });
-    } else if constexpr (IO == FK_IO_C2R) {
+    } else if constexpr (IO == FK_IO_C2R || IO == FK_IO_C2R_RR) {
         /* half-length inverse: Z[k] = E[k] + i O[k],
            E[k] = X[k] + conj X[L-k], O[k] = (X[k] - conj X[L-k]) W_2L^-k,
            z = DFT^-1_L(Z) (unnormalized), y_2m = Re z_m, y_2m+1 = Im z_m.
            Im X[0] and Im X[L] are ignored, as the r2cb codelets do. */
         const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        const R *inr = reinterpret_cast<const R *>(a.in) + bin;
         R *outr = reinterpret_cast<R *>(a.out) + bout;
         const cx<R> *wk = reinterpret_cast<const cx<R> *>(a.aux);
+        const cx<R> *rt = reinterpret_cast<const cx<R> *>(a.rtw);
+        /* fused real-to-real kind; a compile-time FK_RM_NONE in the plain
+           r2c / c2r instantiations */
+        const unsigned rm = (IO == FK_IO_R2C_RR || IO == FK_IO_C2R_RR) ? a.rmode : 0u;
+        constexpr long long N2 = 2LL * L;
+        /* bin k of the fused real-to-real kinds: HC2R reads (r_k, i_k) from
+           the halfcomplex line; REDFT01 forms exp(+i pi k / 2n)(x_k - i x_{n-k})
+           (x_n = 0), RODFT01 the same on the reversed line */
+        auto rbin = [&](int k) -> cx<R> {
+            if (rm == FK_RM_HC2R)
+                return mk(inr[(long long)k * a.is],
+                          (k == 0 || k == L) ? (R)0 : inr[(N2 - k) * a.is]);
+            const bool s = rm == FK_RM_DST3;
+            const long long q0 = s ? N2 - 1 - k : k, q1 = s ? k - 1 : N2 - k;
+            const R xk = inr[q0 * a.is];
+            const R xn = k == 0 ? (R)0 : inr[q1 * a.is];
+            const cx<R> w = rt[2 * k];
+            return mk(w.x * xk - w.y * xn, -(w.x * xn + w.y * xk));
+        };
         /* loads in groups of 8 (compile-time count) so that each group is
            in flight at once, then its LDS stores */
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
@@ -320,7 +393,9 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             sfor<0, cnt>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const int k = u + (g * GK + i) * TPL;
-                if (k <= L) xin[i] = act ? in[(long long)k * a.is] : mk((R)0, (R)0);
+                if (k <= L)
+                    xin[i] = !act ? mk((R)0, (R)0)
+                             : rm == FK_RM_NONE ? in[(long long)k * a.is] : rbin(k);
             });
             sfor<0, cnt>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
@@ -346,7 +421,20 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             /* v = swap(z_k) */
             if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
             if (!act) return;
-            if (fl & FK_REAL_UNIT) {
+            if (rm == FK_RM_DCT3 || rm == FK_RM_DST3) {
+                /* y_2i = v_i (i < L), y_{2i+1} = v_{n-1-i}: v_m goes to 2m
+                   below L, to 2(n-1-m)+1 above; RODFT01 negates odd outputs */
+                const long long m0 = 2LL * k, m1 = 2LL * k + 1;
+                const long long d0 = m0 < L ? 2 * m0 : 2 * (N2 - 1 - m0) + 1;
+                const long long d1 = m1 < L ? 2 * m1 : 2 * (N2 - 1 - m1) + 1;
+                R y0 = v.y, y1 = v.x;
+                if (rm == FK_RM_DST3) {
+                    if (d0 & 1) y0 = -y0;
+                    if (d1 & 1) y1 = -y1;
+                }
+                outr[d0 * a.os] = y0;
+                outr[d1 * a.os] = y1;
+            } else if (fl & FK_REAL_UNIT) {
                 reinterpret_cast<cx<R> *>(outr)[k] = mk(v.y, v.x);
             } else {
                 outr[(2 * k) * a.os] = v.y;

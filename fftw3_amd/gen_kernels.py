@@ -169,6 +169,9 @@ def catalog():
                     if g:
                         entries.append((prec, L, rads, "ROW", io, g))
                         entries += variants(prec, L, rads, "ROW", io, g)
+                        # the same kernel with a fused real-to-real kind
+                        # (runtime rmode; kept out of the plain r2c / c2r code)
+                        entries.append((prec, L, rads, "ROW", io + "_RR", g))
             # fused Bluestein on power-of-two convolution lengths
             if L & (L - 1) == 0 and L >= 4:
                 g = geometry(prec, L, rads, "ROW", "BLUE")

@@ -53,6 +53,22 @@ def test_r2r_1d_sizes(prec, kind):
 
 
 @pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kind,sizes", [(3, [3, 9, 65, 129, 1025, 4097]), (7, [1, 7, 63, 255, 1023, 4095])],
+                         ids=["e00", "o00"])
+def test_r2r_symmetric_extension_sizes(prec, kind, sizes):
+    """REDFT00 / RODFT00 sizes whose logical transform 2(n-1) / 2(n+1) is
+    twice a power of two: the extension is read inside the real kernel."""
+    for n in sizes:
+        x, y, xin = many(prec, kind, n, 4, 300 + n)
+        ref = oracle.r2rn(x.astype(np.float64).reshape(4, n), [kind], axes_batch=1).reshape(-1)
+        assert oracle.rel_l2(y, ref) < TOL[prec], (NAMES[kind], n)
+        assert np.array_equal(xin, x)
+    x, y, _ = many(prec, kind, sizes[-1], 3, 7, inplace=True)
+    ref = oracle.r2rn(x.astype(np.float64).reshape(3, -1), [kind], axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(y, ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
 @pytest.mark.parametrize("kind", ALL, ids=NAMES)
 def test_r2r_inplace(prec, kind):
     for n in (6, 7, 256, 1001):

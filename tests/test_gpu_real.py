@@ -91,10 +91,16 @@ def test_c2r_multid_preserve_input_is_null(eng):
     assert not fa.lib().fftw_plan_dft_c2r(2, n, b.ptr, b.ptr + 4096, fa.FFTW_PRESERVE_INPUT | fa.FFTW_ESTIMATE)
 
 
-def test_r2r_not_provided(eng):
+def test_r2r_planner_through_c_abi(eng):
+    # plan_many_r2r is provided (every kind; numerics in test_gpu_r2r.py);
+    # an unknown kind returns NULL like any unsolvable problem
     b = fa.DeviceBuffer(1024)
     n = (fa.ctypes.c_int * 1)(8)
     k = (fa.ctypes.c_int * 1)(0)
+    p = fa.lib().fftw_plan_many_r2r(1, n, 1, b.ptr, None, 1, 8, b.ptr, None, 1, 8, k, fa.FFTW_ESTIMATE)
+    assert p
+    fa.lib().fftw_destroy_plan(p)
+    k[0] = 42
     assert not fa.lib().fftw_plan_many_r2r(1, n, 1, b.ptr, None, 1, 8, b.ptr, None, 1, 8, k, fa.FFTW_ESTIMATE)
 
 
