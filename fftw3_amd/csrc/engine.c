@@ -1402,7 +1402,27 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
                                                                                 : FK_IO_R2C_RR;
         const long long L = rm == FK_RM_E00 ? n - 1 : rm == FK_RM_O00 ? n + 1 : n / 2;
         const int ok = rm && (rm == FK_RM_E00 || rm == FK_RM_O00 || n % 2 == 0);
-        const fk_kernel *k = ok ? pick_variant(c, prec, L, FK_ROW, io) : 0;
+        const fk_kernel *k = 0;
+        int dC = -1;
+        if (ok && (llabs_(is) != 1 || llabs_(os) != 1)) {
+            /* strided lines: the column mapping over a unit-stride batch dim */
+            long long best = 1LL << 62;
+            for (i = 0; i < nb; ++i) {
+                long long sc = llabs_(b[i].is) + llabs_(b[i].os);
+                if (sc < best && b[i].n >= 2) {
+                    best = sc;
+                    dC = i;
+                }
+            }
+            if (best > 4) dC = -1;
+            if (dC >= 0) k = pick_variant(c, prec, L, FK_COL, io);
+            if (k) {
+                fk_dim t = b[dC];
+                memmove(b + 1, b, sizeof(fk_dim) * dC);
+                b[0] = t;
+            }
+        }
+        if (ok && !k) k = pick_variant(c, prec, L, FK_ROW, io);
         if (k) {
             size_t before = (size_t)P->npass;
             unsigned fl = 0;

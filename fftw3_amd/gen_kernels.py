@@ -170,8 +170,13 @@ def catalog():
                         entries.append((prec, L, rads, "ROW", io, g))
                         entries += variants(prec, L, rads, "ROW", io, g)
                         # the same kernel with a fused real-to-real kind
-                        # (runtime rmode; kept out of the plain r2c / c2r code)
+                        # (runtime rmode; kept out of the plain r2c / c2r
+                        # code), row and column mappings: the column one
+                        # serves r2r dims whose lines are strided
                         entries.append((prec, L, rads, "ROW", io + "_RR", g))
+                        gc = geometry(prec, L, rads, "COL", io)
+                        if gc:
+                            entries.append((prec, L, rads, "COL", io + "_RR", gc))
             # fused Bluestein on power-of-two convolution lengths
             if L & (L - 1) == 0 and L >= 4:
                 g = geometry(prec, L, rads, "ROW", "BLUE")
