@@ -58,12 +58,31 @@ def parse():
     return ap.parse_args()
 
 
+def counter_uniform(count, seed, stream=0):
+    """The device fill's input stream: x_i = 2u - 1, u = top 53 bits of
+    splitmix64(seed ^ (stream << 40) ^ i) / 2^53."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) ^ (np.uint64(stream) << np.uint64(40))) ^ np.arange(count, dtype=np.uint64)
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) - 1.0
+
+
+def rel_l2(a, b):
+    import numpy as np
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def serving_requests(count, seed=5):
     """Request stream: 70% 1-D c2c, 20% 2-D r2c, 10% 3-D c2c (64^3..256^3);
     N log-uniform in 2^8..2^22 with ~10% non-powers of two or primes; 80%
     FP32 / 20% FP64; howmany log-uniform in 1..256, capped at 2^24 points."""
-    import oracle
-    u = (oracle.uniform(8 * count, seed, 11) + 1.0) / 2.0
+    u = (counter_uniform(8 * count, seed, 11) + 1.0) / 2.0
     odd = [3, 5, 7, 9, 15]
     primes = [257, 1009, 4099, 16411, 65537, 262147]
     reqs = []
@@ -145,7 +164,6 @@ def run_serve(args):
     arrivals at `--load` x capacity; reports p50/p99 latency and FFTs/s."""
     import threading
     import fftw3_amd as fa
-    import oracle
     fa.init()
     reqs = serving_requests(args.requests)
     shapes = {}
@@ -190,7 +208,7 @@ def run_serve(args):
     srv = fa.Server(budget, max_rounds=4, per_round=32)
     for r in reqobjs:
         srv.prepare(r)
-    gaps = -np.log(1.0 - (oracle.uniform(len(reqs), 5, 13) + 1.0) / 2.0) / lam
+    gaps = -np.log(1.0 - (counter_uniform(len(reqs), 5, 13) + 1.0) / 2.0) / lam
     arrivals = np.cumsum(gaps)
     ids = []
     done = threading.Event()
@@ -238,9 +256,8 @@ def mixed_sizes(total_log2=28, seed=3):
     """Uneven-size set: N in {1000, 1001, 2187, 4093, 4096, 65536, 2^20},
     batch b_N = floor(exp(U(0, ln 4096))) from the counter-based stream,
     rescaled so that sum N*b_N ~ 2^total_log2 with b_N >= 1."""
-    import oracle
     sizes = [1000, 1001, 2187, 4093, 4096, 65536, 1 << 20]
-    u = (oracle.uniform(len(sizes), seed, 7) + 1.0) / 2.0
+    u = (counter_uniform(len(sizes), seed, 7) + 1.0) / 2.0
     b = [int(math.exp(x * math.log(4096.0))) for x in u]
     tot = sum(n * k for n, k in zip(sizes, b))
     s = float(1 << total_log2) / tot
@@ -253,7 +270,6 @@ def run_mixed(args):
     transform, one batched plan per size, and the persistent work-table
     batch replayed through a hipGraph (FP32 c2c forward)."""
     import fftw3_amd as fa
-    import oracle
     fa.init()
     L = fa.lib()
     L.fftw_amd_set_async(1)
@@ -326,7 +342,7 @@ def run_mixed(args):
     for (n, k), (bi, bo) in zip(mix, bufs):
         x = bi.download(np.complex64, n).astype(np.complex128)
         y = bo.download(np.complex64, n)
-        errs[n] = oracle.rel_l2(y, oracle.fftn(x, -1))
+        errs[n] = rel_l2(y, np.fft.fft(x))
     byts = 2.0 * pts * 8
     res = {}
     for name, t in (("per_call", t1), ("per_size_batched", t2), ("persistent_graph", t3)):
@@ -360,7 +376,6 @@ def main():
         run_precision(args)
         return
     import fftw3_amd as fa
-    import oracle  # checker only: round-trip references and cpu_baseline
 
     fa.init()
     L = fa.lib()
@@ -399,19 +414,19 @@ def main():
 
     # ---- correctness of the timed configuration (untimed) ----
     m = 1 << 20
-    x3 = oracle.uniform(2 * m, 2, 0).view(np.complex128)
+    x3 = counter_uniform(2 * m, 2, 0).view(np.complex128)
     p3f.execute()
     p3b.execute()
     L.fftw_amd_synchronize()
     y3 = b3.download(np.complex128, m)
-    rt3 = oracle.rel_l2(y3 / tot3, x3)
+    rt3 = rel_l2(y3 / tot3, x3)
     b3.fill_uniform("double", 2 * tot3, 2, 0)
     pr.execute()
     pc.execute()
     L.fftw_amd_synchronize()
     xr = rin.download(np.float32, n2 * n2).astype(np.float64)
     yr = rout.download(np.float32, n2 * n2).astype(np.float64)
-    rt2 = oracle.rel_l2(yr / (n2 * n2), xr)
+    rt2 = rel_l2(yr / (n2 * n2), xr)
     log("round trip rel_l2: 3d fp64 %.3e, 2d fp32 %.3e" % (rt3, rt2))
 
     if args.calib_copy:
@@ -493,13 +508,15 @@ def main():
 
     # ---- single-size batched c2c (1-D FP64 N=4096 x 1024, out of place),
     # the configuration of the >= 70 % roofline target; not part of the step
-    c1 = single_size_c2c(fa, L, oracle)
+    c1 = single_size_c2c(fa, L)
+    dct = r2r_dct2(fa, L)
     log("single-size c2c 4096 x 1024 fp64: %.4f ms  %.1f GB/s  frac %.3f  (%s)"
         % (c1["ms"], c1["GB_s"], c1["frac"], c1["passes"][0]))
 
     # ---- CPU baseline: oracle double path on the host (bounded sample) ----
     cpu = None
     if not args.no_cpu:
+        import oracle  # the CPU baseline is the oracle's double-precision path
         cpu = cpu_baseline(oracle, n3, n2)
 
     rec = {
@@ -524,6 +541,7 @@ def main():
             "roundtrip_rel_l2": {"c2c3d_fp64": rt3, "r2c_c2r_2d_fp32": rt2},
             "sub_ms": {k: round(v, 4) for k, v in sub.items()},
             "single_size_c2c_fp64_4096x1024": c1,
+            "r2r_redft10_fp32_2048x2048x16": dct,
         },
         "roofline": {
             "bound": "hbm",
@@ -543,11 +561,11 @@ def main():
     print(json.dumps(rec), flush=True)
 
 
-def single_size_c2c(fa, L, oracle, n=4096, howmany=1024, iters=50):
+def single_size_c2c(fa, L, n=4096, howmany=1024, iters=50):
     """1-D c2c FP64, N=4096, howmany=1024, contiguous, out of place, seed 1
     (FFTW_MEASURE): event-timed mean of `iters` back-to-back executes,
-    algorithmic bytes 2*N*howmany*16, and a parity check of the first
-    transform against the oracle."""
+    algorithmic bytes 2*N*howmany*16, and a check of the first transform
+    against numpy.fft on the same input."""
     import numpy as np
     tot = n * howmany
     a = fa.DeviceBuffer(tot * 16)
@@ -558,8 +576,8 @@ def single_size_c2c(fa, L, oracle, n=4096, howmany=1024, iters=50):
     a.fill_uniform("double", 2 * tot, 1, 0)
     p.execute()
     L.fftw_amd_synchronize()
-    x = oracle.uniform(2 * n, 1, 0).view(np.complex128)
-    err = oracle.rel_l2(b.download(np.complex128, n), oracle.fftn(x, -1))
+    x = a.download(np.complex128, n)
+    err = rel_l2(b.download(np.complex128, n), np.fft.fft(x))
     e0, e1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
     for _ in range(3):
         p.execute()
@@ -573,6 +591,44 @@ def single_size_c2c(fa, L, oracle, n=4096, howmany=1024, iters=50):
     L.fftw_amd_event_destroy(e0)
     L.fftw_amd_event_destroy(e1)
     gbs = 2.0 * tot * 16 / (ms * 1e-3) / 1e9
+    rec = {"ms": round(ms, 5), "GB_s": round(gbs, 1), "frac": round(gbs / PEAK_HBM, 4),
+           "rel_l2_first": err, "passes": p.pass_info()}
+    p.destroy()
+    return rec
+
+
+def r2r_dct2(fa, L, shape=(2048, 2048), howmany=16, iters=20):
+    """REDFT10 (DCT-II) FP32 over every dim of shape x howmany, out of place
+    (FFTW_MEASURE): event-timed mean, algorithmic bytes 2*N*howmany*4, and a
+    check of the first transform against scipy.fft.dctn (type 2, norm None:
+    FFTW's REDFT10 definition) on the same input."""
+    import numpy as np
+    import scipy.fft
+    N = int(np.prod(shape))
+    tot = N * howmany
+    a = fa.DeviceBuffer(tot * 4)
+    b = fa.DeviceBuffer(tot * 4)
+    kinds = [fa.FFTW_REDFT10] * len(shape)
+    p = fa.plan_many_r2r("float", list(shape), howmany, a.ptr, None, 1, N, b.ptr, None, 1, N, kinds,
+                         fa.FFTW_MEASURE)
+    a.fill_uniform("float", tot, 2, 0)
+    p.execute()
+    L.fftw_amd_synchronize()
+    x = a.download(np.float32, N).astype(np.float64).reshape(shape)
+    err = rel_l2(b.download(np.float32, N), scipy.fft.dctn(x, type=2).reshape(-1))
+    e0, e1 = L.fftw_amd_event_create(), L.fftw_amd_event_create()
+    for _ in range(3):
+        p.execute()
+    L.fftw_amd_synchronize()
+    L.fftw_amd_event_record(e0, None)
+    for _ in range(iters):
+        p.execute()
+    L.fftw_amd_event_record(e1, None)
+    L.fftw_amd_synchronize()
+    ms = L.fftw_amd_event_elapsed(e0, e1) / iters
+    L.fftw_amd_event_destroy(e0)
+    L.fftw_amd_event_destroy(e1)
+    gbs = 2.0 * tot * 4 / (ms * 1e-3) / 1e9
     rec = {"ms": round(ms, 5), "GB_s": round(gbs, 1), "frac": round(gbs / PEAK_HBM, 4),
            "rel_l2_first": err, "passes": p.pass_info()}
     p.destroy()
