@@ -1396,10 +1396,12 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
            REDFT10 / RODFT10 with L = n/2, REDFT00 with L = n-1, RODFT00
            with L = n+1; C2R for HC2R / REDFT01 / RODFT01 with L = n/2) */
         static const int rmode[] = {FK_RM_HC, FK_RM_HC2R, FK_RM_DHT, FK_RM_E00, FK_RM_DCT3,
-                                    FK_RM_DCT2, 0, FK_RM_O00, FK_RM_DST3, FK_RM_DST2, 0};
+                                    FK_RM_DCT2, FK_RM_DCT4, FK_RM_O00, FK_RM_DST3, FK_RM_DST2,
+                                    FK_RM_DST4};
         const int rm = rmode[kind];
-        const int io = (rm == FK_RM_HC2R || rm == FK_RM_DCT3 || rm == FK_RM_DST3) ? FK_IO_C2R_RR
-                                                                                : FK_IO_R2C_RR;
+        const int io = (rm == FK_RM_DCT4 || rm == FK_RM_DST4) ? FK_IO_DCT4
+                       : (rm == FK_RM_HC2R || rm == FK_RM_DCT3 || rm == FK_RM_DST3) ? FK_IO_C2R_RR
+                                                                                  : FK_IO_R2C_RR;
         const long long L = rm == FK_RM_E00 ? n - 1 : rm == FK_RM_O00 ? n + 1 : n / 2;
         const int ok = rm && (rm == FK_RM_E00 || rm == FK_RM_O00 || n % 2 == 0);
         const fk_kernel *k = 0;
@@ -1432,7 +1434,7 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
             const void *wk = line_table(prec, 2 * L, L + 1, c->st);
             for (i = 0; i < nb; ++i) unit &= ((io == FK_IO_R2C_RR ? b[i].is : b[i].os) % 2) == 0;
             const int twid = rm == FK_RM_DCT2 || rm == FK_RM_DST2 || rm == FK_RM_DCT3 ||
-                             rm == FK_RM_DST3;
+                             rm == FK_RM_DST3 || rm == FK_RM_DCT4 || rm == FK_RM_DST4;
             if (twid) wt = line_table(prec, 8 * n, 2 * n, c->st);
             if (!wk || (twid && !wt)) {
                 c->err = 1;
@@ -2140,7 +2142,8 @@ int fk_plan_use_mfma(fk_plan *p)
         const fk_kernel *k = ps->k;
         int all16 = 1;
         if (ps->type != FK_PASS_STOCKHAM || k->io == FK_IO_R2C || k->io == FK_IO_C2R ||
-            k->io == FK_IO_BLUE || k->io == FK_IO_R2C_RR || k->io == FK_IO_C2R_RR)
+            k->io == FK_IO_BLUE || k->io == FK_IO_R2C_RR || k->io == FK_IO_C2R_RR ||
+            k->io == FK_IO_DCT4)
             continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (!all16 || (k->L != 16 && k->L != 256 && k->L != 4096)) continue;
@@ -2241,8 +2244,9 @@ double fk_plan_bytes(const fk_plan *p)
 
 static const char *io_name(int io)
 {
-    static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r"};
-    return io >= 0 && io < 7 ? n[io] : "?";
+    static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r",
+                              "dct4"};
+    return io >= 0 && io < 8 ? n[io] : "?";
 }
 
 int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)

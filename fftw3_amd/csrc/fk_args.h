@@ -22,7 +22,8 @@ enum {
     FK_IO_C2R = 3,    /* load n/2+1 bins, half-length inverse, real output */
     FK_IO_BLUE = 4,   /* fused Bluestein convolution of length L */
     FK_IO_R2C_RR = 5, /* FK_IO_R2C with a fused real-to-real kind (rmode) */
-    FK_IO_C2R_RR = 6  /* FK_IO_C2R with a fused real-to-real kind (rmode) */
+    FK_IO_C2R_RR = 6, /* FK_IO_C2R with a fused real-to-real kind (rmode) */
+    FK_IO_DCT4 = 7    /* REDFT11 / RODFT11 of n = 2L reals through an L-point DFT */
 };
 
 /* flags */
@@ -45,7 +46,9 @@ enum { FK_RM_NONE = 0,
        FK_RM_DCT3 = 6,    /* C2R: REDFT01 */
        FK_RM_DST3 = 7,    /* C2R: RODFT01 */
        FK_RM_E00 = 8,     /* R2C of the even extension (length 2L = 2(n-1)): REDFT00 */
-       FK_RM_O00 = 9 };   /* R2C of the odd extension (length 2L = 2(n+1)): RODFT00 */
+       FK_RM_O00 = 9,     /* R2C of the odd extension (length 2L = 2(n+1)): RODFT00 */
+       FK_RM_DCT4 = 10,   /* FK_IO_DCT4: REDFT11 */
+       FK_RM_DST4 = 11 }; /* FK_IO_DCT4: RODFT11 */
 
 typedef struct {
     const void *in;

@@ -258,6 +258,31 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             }
         }
+    } else if constexpr (IO == FK_IO_DCT4) {
+        /* REDFT11 of n = 2L reals: z_m = (x_2m + i x_{n-1-2m}) exp(-i pi (4m+1) / 4n),
+           Z = L-point DFT, c_p = Z_p exp(-i pi p / n), y_2p = 2 Re c_p,
+           y_{n-1-2p} = -2 Im c_p.  RODFT11_k(x) = (-1)^k REDFT11_k(x reversed). */
+        const R *inr = reinterpret_cast<const R *>(a.in) + bin;
+        R *outr = reinterpret_cast<R *>(a.out) + bout;
+        const cx<R> *rt = reinterpret_cast<const cx<R> *>(a.rtw);
+        const bool sn = a.rmode == FK_RM_DST4;
+        constexpr long long N2 = 2LL * L;
+        auto gld = [&](int m) -> cx<R> {
+            if (!act) return mk((R)0, (R)0);
+            const long long q0 = 2LL * m, q1 = N2 - 1 - 2LL * m;
+            const R xa = inr[(sn ? N2 - 1 - q0 : q0) * a.is];
+            const R xb = inr[(sn ? N2 - 1 - q1 : q1) * a.is];
+            return mul(mk(xa, xb), rt[4 * m + 1]);
+        };
+        auto gst = [&](int p, cx<R> v) {
+            if (!act) return;
+            const cx<R> c = mul(v, rt[4 * p]);
+            R ye = (R)2 * c.x, yo = (R)-2 * c.y;
+            if (fl & FK_SCALE) { ye *= (R)a.scale; yo *= (R)a.scale; }
+            outr[(2LL * p) * a.os] = ye;
+            outr[(N2 - 1 - 2LL * p) * a.os] = sn ? -yo : yo;
+        };
+        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
     } else if constexpr (IO == FK_IO_R2C || IO == FK_IO_R2C_RR) {
         /* real line of 2L samples viewed as L complex z_j = x_2j + i x_2j+1;
            after the L-point DFT Z, X[k] = E[k] + W_2L^k O[k] with

@@ -177,6 +177,11 @@ def catalog():
                         gc = geometry(prec, L, rads, "COL", io)
                         if gc:
                             entries.append((prec, L, rads, "COL", io + "_RR", gc))
+                # DCT-IV / DST-IV of 2L reals through the L-point DFT
+                for mode in ("ROW", "COL"):
+                    g = geometry(prec, L, rads, mode, "PLAIN")
+                    if g:
+                        entries.append((prec, L, rads, mode, "DCT4", g))
             # fused Bluestein on power-of-two convolution lengths
             if L & (L - 1) == 0 and L >= 4:
                 g = geometry(prec, L, rads, "ROW", "BLUE")

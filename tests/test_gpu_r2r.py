@@ -83,6 +83,8 @@ def test_r2r_inplace(prec, kind):
                                          ((0, 2, 7), (6, 10, 12)),
                                          ((3, 1, 6), (9, 8, 5)),
                                          ((6, 8), (1, 40)),
+                                         ((6, 10), (64, 32)),
+                                         ((10, 6, 6), (16, 12, 8)),
                                          ((5, 5, 5), (32, 32, 32))])
 @pytest.mark.parametrize("inplace", [False, True])
 def test_r2r_multid(prec, kinds, shape, inplace):
