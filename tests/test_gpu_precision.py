@@ -36,9 +36,14 @@ def test_mfma_variant_accuracy(eng, n, howmany, sign):
     x = gauss_c(n * howmany, 40 + n % 97, np.float32)
     a = fa.DeviceBuffer.from_array(x)
     b = fa.DeviceBuffer(x.nbytes)
-    p = fa.plan_many_dft("float", [n], howmany, a.ptr, None, 1, n, b.ptr, None, 1, n, sign)
+    # FFTW_AMD_MFMA plans radix-16 passes only (and keys its own wisdom), so
+    # the matrix-core kernel serves every pass whatever earlier plans timed
+    p = fa.plan_many_dft("float", [n], howmany, a.ptr, None, 1, n, b.ptr, None, 1, n, sign,
+                         fa.FFTW_ESTIMATE | fa.FFTW_AMD_MFMA)
     switched = fa.lib().fftw_amd_plan_use_mfma(p.h)
     assert switched >= 1
+    rads = [s.split("rad=")[1].split()[0].split("x") for s in p.pass_info()]
+    assert all(set(r) == {"16"} for r in rads), p.pass_info()
     p.execute()
     y = b.download(np.complex64)
     ref = oracle.fftn(x.astype(np.complex128).reshape(howmany, n), sign, axes_batch=1).reshape(-1)
@@ -73,7 +78,8 @@ def test_precision_ladder(eng):
         x = x64.astype(np.complex128 if prec == "double" else np.complex64)
         a = fa.DeviceBuffer.from_array(x)
         b = fa.DeviceBuffer(x.nbytes)
-        p = fa.plan_many_dft(prec, [n], H, a.ptr, None, 1, n, b.ptr, None, 1, n, -1)
+        fl = fa.FFTW_ESTIMATE | (fa.FFTW_AMD_MFMA if mf else 0)
+        p = fa.plan_many_dft(prec, [n], H, a.ptr, None, 1, n, b.ptr, None, 1, n, -1, fl)
         if mf:
             assert fa.lib().fftw_amd_plan_use_mfma(p.h) >= 1
         p.execute()
