@@ -75,8 +75,9 @@ def geometry(prec, L, rads, mode, io, tpl_div=1, nl_mul=1.0):
             return None
     else:
         per_line = (L + 2) * csz
-        # lines of a tile are adjacent in memory: aim for 256 B runs
-        nl = 256 // csz
+        # lines of a tile are adjacent in memory: aim for 256 B runs (of
+        # reals for the real-to-real kernels, which load one real per line)
+        nl = 256 // (csz // 2 if io in ("R2C", "C2R", "DCT4") else csz)
         while nl > 1 and (nl * per_line > LDS_MAX - 1024 or nl * tpl > 1024):
             nl //= 2
         # small transforms: more lines per workgroup for occupancy
@@ -179,7 +180,7 @@ def catalog():
                             entries.append((prec, L, rads, "COL", io + "_RR", gc))
                 # DCT-IV / DST-IV of 2L reals through the L-point DFT
                 for mode in ("ROW", "COL"):
-                    g = geometry(prec, L, rads, mode, "PLAIN")
+                    g = geometry(prec, L, rads, mode, "DCT4")
                     if g:
                         entries.append((prec, L, rads, mode, "DCT4", g))
             # fused Bluestein on power-of-two convolution lengths
