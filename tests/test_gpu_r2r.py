@@ -197,3 +197,88 @@ def test_r2r_large(prec, shape, kinds):
     ref = oracle.r2rn(x.astype(np.float64).reshape(shape), kinds).reshape(-1)
     assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
     p.destroy()
+
+
+R2R_SIZES1 = list(range(1, 21)) + [31, 32, 63, 64, 65, 100, 127, 128, 255, 256, 257, 1000, 1024, 2049]
+R2R_SIZES2 = list(range(1, 17)) + [32, 33, 64]
+R2R_SIZES3 = list(range(1, 10)) + [16]
+
+
+def _layout(rng, shape):
+    """Random physical layout (permuted, padded, one dim reversed) of a
+    logical array: per-dim strides, offset of index 0, buffer span."""
+    nd = len(shape)
+    order = list(rng.permutation(nd)) if rng.random() < 0.4 else list(range(nd))
+    strides = [0] * nd
+    s = 1
+    for d in reversed(order):
+        pad = int(rng.choice([1, 1, 1, 2, 3]))
+        strides[d] = s * pad
+        s *= shape[d] * pad
+        if rng.random() < 0.2:
+            s += int(rng.integers(1, 5))
+    base = 0
+    if rng.random() < 0.25:
+        d = int(rng.integers(0, nd))
+        base += (shape[d] - 1) * strides[d]
+        strides[d] = -strides[d]
+    lo = base + sum(min(0, (n - 1) * st) for n, st in zip(shape, strides))
+    hi = base + sum(max(0, (n - 1) * st) for n, st in zip(shape, strides))
+    return strides, base - lo, hi - lo + 1
+
+
+def _offsets(shape, strides, base):
+    idx = np.full(1, base, np.int64)
+    for n, st in zip(shape, strides):
+        idx = (idx[:, None] + st * np.arange(n, dtype=np.int64)[None, :]).reshape(-1)
+    return idx
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("block", range(6))
+def test_random_r2r_guru_sweep(prec, block):
+    """Seeded sweep in the manner of tests/check.pl over r2r problems: rank
+    1-3, random kinds per dim, howmany 1-3, permuted / padded / reversed
+    (negative-stride) layouts, in place and out of place, ESTIMATE and
+    MEASURE.  Positions the transform must not write stay untouched."""
+    rng = np.random.default_rng(5000 + 31 * block + (prec == "float"))
+    rs = np.dtype(rdt(prec)).itemsize
+    for case in range(12):
+        rank = int(rng.choice([1, 1, 2, 2, 3]))
+        pool = {1: R2R_SIZES1, 2: R2R_SIZES2, 3: R2R_SIZES3}[rank]
+        dims = [int(rng.choice(pool)) for _ in range(rank)]
+        kinds = [int(rng.integers(0, 11)) for _ in range(rank)]
+        kinds = [5 if (k == 3 and n == 1) else k for k, n in zip(kinds, dims)]
+        H = int(rng.integers(1, 4))
+        shape = [H] + dims
+        inplace = rng.random() < 0.3
+        si, bi, span_i = _layout(rng, shape)
+        so, bo, span_o = (si, bi, span_i) if inplace else _layout(rng, shape)
+        measure = rng.random() < 0.2
+        N = int(np.prod(shape))
+        x = oracle.uniform(N, 9000 + case, block).astype(rdt(prec))
+        buf_i = np.zeros(span_i, rdt(prec))
+        buf_i[_offsets(shape, si, bi)] = x
+        dI = fa.DeviceBuffer.from_array(buf_i)
+        dO = dI if inplace else fa.DeviceBuffer.from_array(np.zeros(span_o, rdt(prec)))
+        gd = [(shape[k], si[k], so[k]) for k in range(1, len(shape))]
+        hd = [(shape[0], si[0], so[0])]
+        what = (shape, kinds, si, so, bi, bo, inplace, measure)
+        print("r2r sweep", prec, block, case, what, flush=True)
+        p = fa.plan_guru_r2r(prec, gd, hd, dI.ptr + bi * rs, dO.ptr + bo * rs, kinds,
+                             fa.FFTW_MEASURE if measure else fa.FFTW_ESTIMATE)
+        if measure:
+            dI.upload(buf_i)
+            if not inplace:
+                dO.upload(np.zeros(span_o, rdt(prec)))
+        p.execute()
+        y = dO.download(rdt(prec))
+        p.destroy()
+        ref = oracle.r2rn(x.astype(np.float64).reshape(shape), kinds, axes_batch=1).reshape(-1)
+        oo = _offsets(shape, so, bo)
+        assert oracle.rel_l2(y[oo], ref) <= TOL[prec], what
+        if not inplace:
+            m = np.ones(span_o, bool)
+            m[oo] = False
+            assert not np.any(y[m]), what
+            assert np.array_equal(dI.download(rdt(prec)), buf_i), ("input changed", what)
