@@ -269,6 +269,21 @@ static const fk_kernel *find_kernel(int prec, long long L, int mode, int io)
 
 static int has_single(int prec, long long n) { return find_kernel(prec, n, FK_ROW, FK_IO_PLAIN) != 0; }
 
+/* the catalog variant of (prec, L, mode, io) whose stages are all radix 16
+   (the matrix-core substitution needs them), or NULL */
+static const fk_kernel *find_kernel_r16(int prec, long long L, int mode, int io)
+{
+    int i, j;
+    for (i = 0; i < g_nreg; ++i) {
+        const fk_kernel *k = g_reg + i;
+        int all16 = k->nrad > 0;
+        if (k->prec != prec || k->L != L || k->mode != mode || k->io != io) continue;
+        for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
+        if (all16) return k;
+    }
+    return 0;
+}
+
 /* n decomposable into catalog passes without Bluestein */
 static int is_pow16(long long n)
 {
@@ -768,6 +783,11 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
         if (m == 1) k = pick_variant(c, prec, n, FK_ROW, FK_IO_TOUT);
         else if (m == 2) k = pick_variant(c, prec, n, FK_COL, FK_IO_PLAIN);
         else k = pick_variant(c, prec, n, FK_ROW, FK_IO_PLAIN);
+        if (c->r16) {
+            /* matrix-core plans: the all-radix-16 variant of the mapping */
+            const fk_kernel *k16 = find_kernel_r16(prec, n, k->mode, k->io);
+            if (k16) k = k16;
+        }
         emit_stockham(c, k, is, os, nb, b, ib, io, ob, oo, fl, cs, cs);
         return c->err ? -1 : 0;
     }

@@ -54,7 +54,7 @@ static unsigned long long catalog_hash(void)
         const fk_kernel *k = fk_registry(&n);
         /* the planner's decision sequence is part of what wisdom means:
            the tag changes whenever decisions are added or reordered */
-        h = fk_fnv1a("fk-catalog-2/planner-chunk", 26, 0);
+        h = fk_fnv1a("fk-catalog-3/planner-chunk", 26, 0);
         for (i = 0; i < n; ++i) {
             int v[21], j, m = 0;
             v[m++] = k[i].prec;

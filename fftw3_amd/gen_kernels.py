@@ -134,6 +134,13 @@ def variants(prec, L, rads, mode, io, g0):
         cands += [(rads, 1, 4.0), (rads, 2, 4.0), (rads, 2, 8.0)]
     if list(reversed(rads)) != list(rads):
         cands.append((list(reversed(rads)), 1, 1.0))
+    if L >= 1024 and mode == "ROW" and io == "PLAIN":
+        # radix-8 stages: one more LDS stage, twice the threads per line
+        # (latency hiding when LDS allows only two workgroups per CU)
+        k = L.bit_length() - 1
+        r8 = [8] * (k // 3) + ([1 << (k % 3)] if k % 3 else [])
+        if r8 != list(rads):
+            cands.append((r8, 1, 1.0))
     for r, td, nm in cands:
         g = geometry(prec, L, r, mode, io, tpl_div=td, nl_mul=nm)
         if not g or g["tpl"] < 16:
@@ -159,10 +166,20 @@ def catalog():
             for mode, io in (("ROW", "PLAIN"), ("COL", "PLAIN"), ("ROW", "TOUT")):
                 if io == "TOUT" and (L < 16 or not fam):
                     continue
-                g = geometry(prec, L, rads, mode, io)
+                r0 = rads
+                if mode == "ROW" and io == "PLAIN" and L in (2048, 4096):
+                    # one line per workgroup: radix-8 stages (twice the
+                    # threads per line) measured faster than radix 16 for
+                    # both precisions; radix 16 stays a tuner alternative
+                    r0 = [8] * ((L.bit_length() - 1) // 3) + ([4] if L == 2048 else [])
+                g = geometry(prec, L, r0, mode, io)
                 if g:
-                    entries.append((prec, L, rads, mode, io, g))
-                    entries += variants(prec, L, rads, mode, io, g)
+                    entries.append((prec, L, r0, mode, io, g))
+                    entries += variants(prec, L, r0, mode, io, g)
+                    if r0 is not rads:
+                        g16 = geometry(prec, L, rads, mode, io)
+                        if g16:
+                            entries.append((prec, L, rads, mode, io, g16))
             # half-length kernels for even real transforms (n = 2L)
             if (fam or L <= 64) and L <= 8192:
                 for io in ("R2C", "C2R"):

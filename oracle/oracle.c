@@ -115,7 +115,7 @@ static void dft_##SFX(int rank, const orc_iodim *dims, int hrank,            \
                       int sign)                                               \
 {                                                                             \
     long total = prod_n(rank, dims), H = prod_n(hrank, hdims), i;           \
-    long n[33];                                                               \
+    long n[33] = {0};                                                         \
     int d;                                                                    \
     cpx_##SFX *buf;                                                           \
     if (H == 0) return;                                                       \
@@ -178,7 +178,7 @@ static void c2r_##SFX(int rank, const orc_iodim *dims, int hrank,            \
                       const orc_iodim *hdims, const double *in, double *out)  \
 {                                                                             \
     long H = prod_n(hrank, hdims), i, rows, nl, nh, tot_h;                    \
-    long n[33];                                                               \
+    long n[33] = {0};                                                         \
     int d;                                                                    \
     cpx_##SFX *buf;                                                           \
     orc_iodim id[33];                                                         \
