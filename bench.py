@@ -363,7 +363,7 @@ def run_mixed(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU only; this build has no multi-GPU path "
+        print("bench.py measures one GPU only; this engine has no multi-GPU path "
               "(requested --gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == "mixed":

@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py import this module, and only as the checker.
+cpu_baseline leg of bench.py import this module, and only to check results.
 The product library (fftw3_amd) never imports or links it.
 
 The oracle restates FFTW 3.3's DFT semantics from the mathematics; see

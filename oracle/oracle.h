@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the DFT hot path.  TEST INFRASTRUCTURE ONLY: nothing in the
  * product library links this; only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load it, and only as the checker.
+ * cpu_baseline leg of bench.py load it, and only to check results.
  *
  * It restates, from the mathematics, what FFTW 3.3 computes on the
  * plan/execute path:

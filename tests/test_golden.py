@@ -1,6 +1,6 @@
 """Committed oracle fixtures (tests/golden/golden.npz).
 
-CPU: the oracle still reproduces its frozen outputs (guards the checker).
+CPU: the oracle still reproduces its frozen outputs (guards the oracle itself).
 GPU: the engine matches the frozen outputs through the C ABI, with no live
 oracle involved.
 """

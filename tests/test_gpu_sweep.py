@@ -9,7 +9,7 @@
   the transform must not write are checked untouched.
 * One plan executed concurrently from several host threads on different
   arrays (only fftw_execute* is thread-safe, doc/threads.texi:233-246).
-* The BASELINE configurations at full size, through properties that do not
+* The benchmark configurations at full size, through properties that do not
   need a full-size oracle: round trip backward(forward x) = N x (C2a 512^3
   FP64, C2b 2048^2 x 64 FP32 r2c/c2r), linearity, and exact sampled output
   bins from the oracle's direct DFT (C4, N = 2^24).
