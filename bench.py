@@ -645,7 +645,8 @@ def kernel_symbol(info):
     rads = kv["rad"].split("x")
     real = "double" if f[1] == "f64" else "float"
     mode = 0 if f[4] == "row" else 1
-    io = {"plain": 0, "tout": 1, "r2c": 2, "c2r": 3, "bluestein": 4, "r2c-r2r": 5, "c2r-r2r": 6, "dct4": 7}[f[5]]
+    io = {"plain": 0, "tout": 1, "r2c": 2, "c2r": 3, "bluestein": 4, "r2c-r2r": 5, "c2r-r2r": 6, "dct4": 7,
+          "r2c-full": 8, "c2r-full": 9}[f[5]]
     return "void fk::stockham<%s, %s, %d, %d, %s, fk::RadList<%s>, %s>(fk_args)" % (
         real, kv["nl"], mode, io, rads[0], ", ".join(rads), kv["tpl"])
 

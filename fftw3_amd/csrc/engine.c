@@ -1135,6 +1135,14 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         }
         if (unit) P->align_need = (unsigned)(2 * rs);
         desc_add(P, "(r2c-fused L=%lld)", L);
+    } else if ((k = pick_variant(c, prec, nl, FK_ROW, FK_IO_R2C_FULL)) != 0) {
+        /* one pass: full-length DFT of the real line, half the bins stored */
+        size_t before = (size_t)P->npass;
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, c->bin, 0, c->bout, 0,
+                      0, rs, cs);
+        for (; before < (size_t)P->npass; ++before)
+            P->pass[before].bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+        desc_add(P, "(r2c-full n=%lld)", nl);
     } else {
         /* widen -> full complex DFT along the last dim -> copy n/2+1 bins */
         long long H = batch_count(nb, b), s;
@@ -1247,6 +1255,14 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         }
         if (unit) P->align_need = (unsigned)(2 * rs);
         desc_add(P, "(c2r-fused L=%lld)", L);
+    } else if ((k = pick_variant(c, prec, nl, FK_ROW, FK_IO_C2R_FULL)) != 0) {
+        /* one pass: Hermitian expansion on load, real part on store */
+        size_t before = (size_t)P->npass;
+        emit_stockham(c, k, pb->dims[r - 1].is, pb->dims[r - 1].os, nb, b, c->bin, 0, c->bout, 0,
+                      0, cs, rs);
+        for (; before < (size_t)P->npass; ++before)
+            P->pass[before].bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+        desc_add(P, "(c2r-full n=%lld)", nl);
     } else {
         /* Hermitian expansion -> full complex inverse -> real part */
         long long H = batch_count(nb, b), s;
@@ -2163,7 +2179,7 @@ int fk_plan_use_mfma(fk_plan *p)
         int all16 = 1;
         if (ps->type != FK_PASS_STOCKHAM || k->io == FK_IO_R2C || k->io == FK_IO_C2R ||
             k->io == FK_IO_BLUE || k->io == FK_IO_R2C_RR || k->io == FK_IO_C2R_RR ||
-            k->io == FK_IO_DCT4)
+            k->io == FK_IO_DCT4 || k->io == FK_IO_R2C_FULL || k->io == FK_IO_C2R_FULL)
             continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (!all16 || (k->L != 16 && k->L != 256 && k->L != 4096)) continue;
@@ -2265,8 +2281,8 @@ double fk_plan_bytes(const fk_plan *p)
 static const char *io_name(int io)
 {
     static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r",
-                              "dct4"};
-    return io >= 0 && io < 8 ? n[io] : "?";
+                              "dct4", "r2c-full", "c2r-full"};
+    return io >= 0 && io < 10 ? n[io] : "?";
 }
 
 int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)

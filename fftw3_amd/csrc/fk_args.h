@@ -23,7 +23,9 @@ enum {
     FK_IO_BLUE = 4,   /* fused Bluestein convolution of length L */
     FK_IO_R2C_RR = 5, /* FK_IO_R2C with a fused real-to-real kind (rmode) */
     FK_IO_C2R_RR = 6, /* FK_IO_C2R with a fused real-to-real kind (rmode) */
-    FK_IO_DCT4 = 7    /* REDFT11 / RODFT11 of n = 2L reals through an L-point DFT */
+    FK_IO_DCT4 = 7,   /* REDFT11 / RODFT11 of n = 2L reals through an L-point DFT */
+    FK_IO_R2C_FULL = 8, /* real line of n = L samples, L-point DFT, bins k <= L/2 stored */
+    FK_IO_C2R_FULL = 9  /* bins k <= L/2, Hermitian L-point inverse, real part stored */
 };
 
 /* flags */

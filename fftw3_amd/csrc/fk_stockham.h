@@ -258,6 +258,46 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             }
         }
+    } else if constexpr (IO == FK_IO_R2C_FULL) {
+        /* r2c of a length-L real line whose half length has no kernel (odd
+           L): the full L-point DFT of (x_j, 0); bins k <= L/2 are stored */
+        const R *inr = reinterpret_cast<const R *>(a.in) + bin;
+        cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+        auto gld = [&](int j) -> cx<R> {
+            return act ? mk(inr[(long long)j * a.is], (R)0) : mk((R)0, (R)0);
+        };
+        auto gst = [&](int k, cx<R> v) {
+            if (act && 2 * k <= L) {
+                if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+                out[bout + (long long)k * a.os] = v;
+            }
+        };
+        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+    } else if constexpr (IO == FK_IO_C2R_FULL) {
+        /* c2r of length L from bins k <= L/2: X_{L-k} = conj X_k (Im X_0 and,
+           for even L, Im X_{L/2} ignored); the inverse DFT as a forward DFT
+           of swap(X), whose real output part is Im of the result */
+        const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        R *outr = reinterpret_cast<R *>(a.out) + bout;
+        auto gld = [&](int j) -> cx<R> {
+            if (!act) return mk((R)0, (R)0);
+            cx<R> v;
+            if (2 * j <= L) {
+                v = in[(long long)j * a.is];
+                if (j == 0 || 2 * j == L) v.y = (R)0;
+            } else {
+                v = in[(long long)(L - j) * a.is];
+                v.y = -v.y;
+            }
+            return swp(v);
+        };
+        auto gst = [&](int k, cx<R> v) {
+            if (!act) return;
+            R y = v.y;
+            if (fl & FK_SCALE) y *= (R)a.scale;
+            outr[(long long)k * a.os] = y;
+        };
+        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
     } else if constexpr (IO == FK_IO_DCT4) {
         /* REDFT11 of n = 2L reals: z_m = (x_2m + i x_{n-1-2m}) exp(-i pi (4m+1) / 4n),
            Z = L-point DFT, c_p = Z_p exp(-i pi p / n), y_2p = 2 Re c_p,

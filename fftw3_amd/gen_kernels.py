@@ -195,6 +195,13 @@ def catalog():
                         gc = geometry(prec, L, rads, "COL", io)
                         if gc:
                             entries.append((prec, L, rads, "COL", io + "_RR", gc))
+            # full-length real kernels for odd lengths (no half-length form)
+            if L % 2 == 1 and L <= 8192:
+                for io in ("R2C_FULL", "C2R_FULL"):
+                    g = geometry(prec, L, rads, "ROW", "PLAIN")
+                    if g:
+                        entries.append((prec, L, rads, "ROW", io, g))
+            if (fam or L <= 64) and L <= 8192:
                 # DCT-IV / DST-IV of 2L reals through the L-point DFT
                 for mode in ("ROW", "COL"):
                     g = geometry(prec, L, rads, mode, "DCT4")
