@@ -1103,6 +1103,16 @@ static int plan_copy(ctx *c, int vrank, const fk_dim *vdims, int op, size_t esz)
     return 0;
 }
 
+/* every batch stride of the real side (input: 0, output: 1) even, so the
+   real lines can be viewed as complex ones */
+static int real_pairs(int nb, const fk_dim *b, int out)
+{
+    int i;
+    for (i = 0; i < nb; ++i)
+        if ((out ? b[i].os : b[i].is) % 2) return 0;
+    return 1;
+}
+
 static int plan_r2c(ctx *c, const fk_problem *pb)
 {
     fk_plan *P = c->P;
@@ -1143,6 +1153,39 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         for (; before < (size_t)P->npass; ++before)
             P->pass[before].bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
         desc_add(P, "(r2c-full n=%lld)", nl);
+    } else if (nl % 2 == 0 && nl >= 4 && pb->dims[r - 1].is == 1 &&
+               ((uintptr_t)pb->in % (2 * rs)) == 0 && real_pairs(nb, b, 0)) {
+        /* even n without a half-length kernel: the line read as L complex
+           z_j = x_2j + i x_2j+1 (no copy), the engine's L-point DFT into
+           staging, then X[k] = E[k] + W_n^k O[k] for k <= L */
+        long long H = batch_count(nb, b), s = L;
+        fk_dim cb[FK_MAXRANK * 2], pbat[FK_MAXRANK * 2];
+        fk_pass proto;
+        const void *wk = line_table(prec, nl, L + 1, c->st);
+        if (!wk) { c->err = 1; return -1; }
+        need_scratch(c, 6, cs * (size_t)L * (size_t)H);
+        for (i = 0; i < nb; ++i) {
+            cb[i].n = pbat[i].n = b[i].n;
+            cb[i].is = b[i].is / 2;
+            cb[i].os = pbat[i].is = s;
+            pbat[i].os = b[i].os;
+            s *= b[i].n;
+        }
+        if (plan_c2c_1d(c, L, 1, 1, nb, cb, c->bin, 0, FK_BUF_S0 + 6, 0, 0, 0, 0)) return -1;
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_R2C_POST;
+        proto.e.n = L + 1;
+        proto.e.p0 = L;
+        proto.e.is = 1;
+        proto.e.os = pb->dims[r - 1].os;
+        proto.e.aux = wk;
+        proto.ibuf = FK_BUF_S0 + 6;
+        proto.obuf = c->bout;
+        proto.bytes = (double)H * ((double)L + (double)nh) * cs;
+        emit_loops(c, &proto, nb, pbat, cs, cs);
+        P->align_need = (unsigned)(2 * rs);
+        desc_add(P, "(r2c-half n=%lld)", nl);
     } else {
         /* widen -> full complex DFT along the last dim -> copy n/2+1 bins */
         long long H = batch_count(nb, b), s;
@@ -1263,6 +1306,39 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         for (; before < (size_t)P->npass; ++before)
             P->pass[before].bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
         desc_add(P, "(c2r-full n=%lld)", nl);
+    } else if (nl % 2 == 0 && nl >= 4 && pb->dims[r - 1].os == 1 &&
+               ((uintptr_t)pb->out % (2 * rs)) == 0 && real_pairs(nb, b, 1)) {
+        /* even n without a half-length kernel: Z[k] = E[k] + i O[k] into
+           staging, then the engine's L-point inverse DFT written straight to
+           the output viewed as L complex (y_2m = Re z_m, y_2m+1 = Im z_m) */
+        long long H = batch_count(nb, b), s = L;
+        fk_dim pbat[FK_MAXRANK * 2], cb[FK_MAXRANK * 2];
+        fk_pass proto;
+        const void *wk = line_table(prec, nl, L + 1, c->st);
+        if (!wk) { c->err = 1; return -1; }
+        need_scratch(c, 6, cs * (size_t)L * (size_t)H);
+        for (i = 0; i < nb; ++i) {
+            pbat[i].n = cb[i].n = b[i].n;
+            pbat[i].is = b[i].is;
+            pbat[i].os = cb[i].is = s;
+            cb[i].os = b[i].os / 2;
+            s *= b[i].n;
+        }
+        memset(&proto, 0, sizeof(proto));
+        proto.type = FK_PASS_EW;
+        proto.e.op = FK_EW_C2R_PRE;
+        proto.e.n = L;
+        proto.e.p0 = L;
+        proto.e.is = pb->dims[r - 1].is;
+        proto.e.os = 1;
+        proto.e.aux = wk;
+        proto.ibuf = c->bin;
+        proto.obuf = FK_BUF_S0 + 6;
+        proto.bytes = (double)H * ((double)nh + (double)L) * cs;
+        emit_loops(c, &proto, nb, pbat, cs, cs);
+        if (plan_c2c_1d(c, L, 1, 1, nb, cb, FK_BUF_S0 + 6, 0, c->bout, 0, 1, 1, 0)) return -1;
+        P->align_need = (unsigned)(2 * rs);
+        desc_add(P, "(c2r-half n=%lld)", nl);
     } else {
         /* Hermitian expansion -> full complex inverse -> real part */
         long long H = batch_count(nb, b), s;

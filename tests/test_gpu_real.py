@@ -59,7 +59,7 @@ def r2c_c2r(prec, shape, howmany, seed=2, inplace=False):
 
 @pytest.mark.parametrize("prec", ["double", "float"])
 @pytest.mark.parametrize("n", list(range(1, 65)) + [100, 128, 1000, 1001, 2048, 4096, 8192, 16384,
-                                                   4093, 65536, 1 << 20])
+                                                   4093, 65536, 1 << 20, 130, 2018, 8186, 32768 * 3])
 def test_r2c_c2r_1d(eng, prec, n):
     e_fwd, e_bwd, e_rt = r2c_c2r(prec, [n], 3, seed=n)
     assert e_fwd <= TOL[prec], e_fwd
@@ -78,7 +78,7 @@ def test_r2c_c2r_multid(eng, prec, shape):
 
 
 @pytest.mark.parametrize("prec", ["double", "float"])
-@pytest.mark.parametrize("shape", [(16,), (15,), (8, 12), (64, 64)])
+@pytest.mark.parametrize("shape", [(16,), (15,), (8, 12), (64, 64), (2018,), (4, 65536)])
 def test_r2c_c2r_inplace(eng, prec, shape):
     e_fwd, e_bwd, e_rt = r2c_c2r(prec, shape, 2, seed=4, inplace=True)
     assert e_fwd <= TOL[prec], e_fwd
