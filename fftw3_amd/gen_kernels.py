@@ -134,7 +134,7 @@ def variants(prec, L, rads, mode, io, g0):
         cands += [(rads, 1, 4.0), (rads, 2, 4.0), (rads, 2, 8.0)]
     if list(reversed(rads)) != list(rads):
         cands.append((list(reversed(rads)), 1, 1.0))
-    if L >= 1024 and mode == "ROW" and io == "PLAIN":
+    if L >= 1024 and mode == "ROW" and io in ("PLAIN", "R2C", "C2R"):
         # radix-8 stages: one more LDS stage, twice the threads per line
         # (latency hiding when LDS allows only two workgroups per CU)
         k = L.bit_length() - 1
