@@ -282,3 +282,47 @@ def test_random_r2r_guru_sweep(prec, block):
             m[oo] = False
             assert not np.any(y[m]), what
             assert np.array_equal(dI.download(rdt(prec)), buf_i), ("input changed", what)
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_r2r_flops_print_wisdom_copy(prec):
+    """fftw_flops (libbench2/mflops.c convention: 2.5 N log2 N for r2r),
+    print_plan names the kinds, MEASURE wisdom survives export / forget /
+    import and makes WISDOM_ONLY planning succeed, copy_plan refcounts."""
+    L = fa.lib()
+    P = fa.prefix(prec)
+    n0, n1, h = 64, 48, 4
+    N = n0 * n1
+    x = inputs_r(N * h, 71, prec)
+    a, o = fa.DeviceBuffer.from_array(x), fa.DeviceBuffer(x.nbytes)
+    kinds = [fa.FFTW_REDFT10, fa.FFTW_RODFT11]
+    getattr(L, P + "forget_wisdom")()
+    p = fa.plan_many_r2r(prec, [n0, n1], h, a.ptr, None, 1, N, o.ptr, None, 1, N, kinds, fa.FFTW_MEASURE)
+    add, mul, fma = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    getattr(L, P + "flops")(p.h, ctypes.byref(add), ctypes.byref(mul), ctypes.byref(fma))
+    nominal = 2.5 * N * np.log2(N) * h
+    assert abs(add.value + mul.value + 2 * fma.value - nominal) <= 1e-6 * nominal
+    desc = p.sprint()
+    assert "r2r" in desc and "e10" in desc and "o11" in desc
+    q = fa.Plan(getattr(L, P + "copy_plan")(p.h), prec)
+    p.destroy()
+    a.upload(x)
+    q.execute()
+    ref = oracle.r2rn(x.astype(np.float64).reshape(h, n0, n1), kinds, axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
+    q.destroy()
+    ptr = L.fftw_export_wisdom_to_string()
+    s = ctypes.cast(ptr, ctypes.c_char_p).value
+    L._libc.free(ptr)
+    assert s.count(b"(#x") >= 1
+    getattr(L, P + "forget_wisdom")()
+    with pytest.raises(RuntimeError):
+        fa.plan_many_r2r(prec, [n0, n1], h, a.ptr, None, 1, N, o.ptr, None, 1, N, kinds,
+                         fa.FFTW_ESTIMATE | fa.FFTW_WISDOM_ONLY)
+    assert L.fftw_import_wisdom_from_string(s) == 1
+    w = fa.plan_many_r2r(prec, [n0, n1], h, a.ptr, None, 1, N, o.ptr, None, 1, N, kinds,
+                         fa.FFTW_ESTIMATE | fa.FFTW_WISDOM_ONLY)
+    a.upload(x)
+    w.execute()
+    assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
+    w.destroy()
