@@ -326,3 +326,34 @@ def test_r2r_flops_print_wisdom_copy(prec):
     w.execute()
     assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
     w.destroy()
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_r2r_edge_shapes(prec):
+    """howmany = 0 (nothing written), rank 3 with two howmany dims (more
+    batch dims than one launch addresses), a length with a large prime
+    factor (8193 = 3 * 2731) and a 2^18 line (half-length real transform
+    through a four-step)."""
+    b = fa.DeviceBuffer(4096)
+    b.upload(np.full(1024, 5, np.float32))
+    p = fa.plan_guru_r2r(prec, [(8, 1, 1)], [(0, 8, 8)], b.ptr, b.ptr, [5])
+    p.execute()
+    assert np.all(b.download(np.float32) == 5)
+    p.destroy()
+    shape = (2, 3, 4, 5, 6)          # (h1, h0, n0, n1, n2)
+    N = int(np.prod(shape))
+    x = inputs_r(N, 81, prec)
+    a, o = fa.DeviceBuffer.from_array(x), fa.DeviceBuffer(x.nbytes)
+    dims = [(4, 30, 30), (5, 6, 6), (6, 1, 1)]
+    hd = [(3, 120, 120), (2, 360, 360)]
+    kinds = [7, 0, 6]
+    p = fa.plan_guru_r2r(prec, dims, hd, a.ptr, o.ptr, kinds)
+    p.execute()
+    ref = oracle.r2rn(x.astype(np.float64).reshape(shape), kinds, axes_batch=2).reshape(-1)
+    assert oracle.rel_l2(o.download(rdt(prec)), ref) < TOL[prec]
+    p.destroy()
+    for kind, n, h in ((5, 8193, 2), (4, 8193, 2), (10, 8193, 2), (5, 1 << 18, 2), (1, 1 << 18, 2)):
+        x, y, xin = many(prec, kind, n, h, 90 + kind)
+        ref = oracle.r2rn(x.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
+        assert oracle.rel_l2(y, ref) < TOL[prec], (NAMES[kind], n)
+        assert np.array_equal(xin, x)
