@@ -235,7 +235,7 @@ def _offsets(shape, strides, base):
 
 
 @pytest.mark.parametrize("prec", ["double", "float"])
-@pytest.mark.parametrize("block", range(6))
+@pytest.mark.parametrize("block", range(10))
 def test_random_r2r_guru_sweep(prec, block):
     """Seeded sweep in the manner of tests/check.pl over r2r problems: rank
     1-3, random kinds per dim, howmany 1-3, permuted / padded / reversed
@@ -243,7 +243,7 @@ def test_random_r2r_guru_sweep(prec, block):
     MEASURE.  Positions the transform must not write stay untouched."""
     rng = np.random.default_rng(5000 + 31 * block + (prec == "float"))
     rs = np.dtype(rdt(prec)).itemsize
-    for case in range(12):
+    for case in range(15):
         rank = int(rng.choice([1, 1, 2, 2, 3]))
         pool = {1: R2R_SIZES1, 2: R2R_SIZES2, 3: R2R_SIZES3}[rank]
         dims = [int(rng.choice(pool)) for _ in range(rank)]
@@ -266,7 +266,7 @@ def test_random_r2r_guru_sweep(prec, block):
         what = (shape, kinds, si, so, bi, bo, inplace, measure)
         print("r2r sweep", prec, block, case, what, flush=True)
         p = fa.plan_guru_r2r(prec, gd, hd, dI.ptr + bi * rs, dO.ptr + bo * rs, kinds,
-                             fa.FFTW_MEASURE if measure else fa.FFTW_ESTIMATE)
+                             fa.FFTW_MEASURE if measure else fa.FFTW_ESTIMATE, wide=bool(case % 2))
         if measure:
             dI.upload(buf_i)
             if not inplace:
