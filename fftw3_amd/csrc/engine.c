@@ -1554,6 +1554,15 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
                 return -1;
             }
             if (unit) fl |= FK_REAL_UNIT;
+            if (prec == 1 && k->mode == FK_ROW && (rm == FK_RM_DCT2 || rm == FK_RM_DST2) && is == 1 &&
+                ((uintptr_t)iptr % 16) == 0) {
+                int quad = 1;
+                for (i = 0; i < nb; ++i) quad &= (b[i].is % 4) == 0;
+                if (quad) {
+                    fl |= FK_REAL_QUAD;
+                    if (P->align_need < 16) P->align_need = 16;
+                }
+            }
             emit_stockham(c, k, is, os, nb, b, ib, 0, ob, 0, fl, rs, rs);
             for (; before < (size_t)P->npass; ++before) {
                 fk_pass *p = P->pass + before;

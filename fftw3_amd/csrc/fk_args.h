@@ -35,6 +35,8 @@ enum {
                              i0 by W_N^(k*i0) (four-step twiddle) */
 #define FK_SCALE     8u   /* multiply output by scale */
 #define FK_REAL_UNIT 16u  /* R2C: real input stride is 1 / C2R: output stride 1 */
+#define FK_REAL_QUAD 32u  /* FP32 real input stride 1, lines 16-byte aligned: the
+                             reordered loads of REDFT10 / RODFT10 read quads */
 
 /* real-to-real kinds fused into the FK_IO_R2C_RR / FK_IO_C2R_RR kernels (rmode):
    the real line is read or written in the kind's order, with the kind's

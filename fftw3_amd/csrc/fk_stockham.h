@@ -355,6 +355,21 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         };
         auto gld = [&](int j) -> cx<R> {
             if (!act) return mk((R)0, (R)0);
+            if constexpr (sizeof(R) == 4) {
+                /* v_2j, v_2j+1 are x_4j, x_4j+2 (first half) or x_{2n-4j-1},
+                   x_{2n-4j-3} (second half): one 16-byte quad each */
+                if ((rm == FK_RM_DCT2 || rm == FK_RM_DST2) && (fl & FK_REAL_QUAD)) {
+                    const float4 *q4 = reinterpret_cast<const float4 *>(inr);
+                    if (2 * j + 1 < L) {
+                        const float4 q = q4[j];
+                        return mk(q.x, q.z);
+                    }
+                    if (2 * j >= L) {
+                        const float4 q = q4[L - 1 - j];
+                        return rm == FK_RM_DST2 ? mk(-q.w, -q.y) : mk(q.w, q.y);
+                    }
+                }
+            }
             if (rm == FK_RM_DCT2 || rm == FK_RM_DST2) return mk(vld(2 * j), vld(2 * j + 1));
             if (rm == FK_RM_E00 || rm == FK_RM_O00) return mk(eld(2 * j), eld(2 * j + 1));
             if (fl & FK_REAL_UNIT) return reinterpret_cast<const cx<R> *>(inr)[j];
