@@ -391,6 +391,14 @@ static void need_scratch(ctx *c, int slot, size_t bytes)
     if (P->nscratch < slot + 1) P->nscratch = slot + 1;
 }
 
+/* a pass reads or writes buffer `buf` in units of `a` bytes: the array bound
+   to it at execute must keep that alignment (staging scratch always does) */
+static void need_align(ctx *c, int buf, unsigned a)
+{
+    if (buf == FK_BUF_IN && c->P->align_in < a) c->P->align_in = a;
+    if (buf == FK_BUF_OUT && c->P->align_out < a) c->P->align_out = a;
+}
+
 static long long llabs_(long long x) { return x < 0 ? -x : x; }
 
 /* Drop unit dims; stable. */
@@ -1143,7 +1151,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
             p->a.aux = wk;
             p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
         }
-        if (unit) P->align_need = (unsigned)(2 * rs);
+        if (unit) need_align(c, c->bin, (unsigned)(2 * rs));
         desc_add(P, "(r2c-fused L=%lld)", L);
     } else if ((k = pick_variant(c, prec, nl, FK_ROW, FK_IO_R2C_FULL)) != 0) {
         /* one pass: full-length DFT of the real line, half the bins stored */
@@ -1184,7 +1192,7 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
         proto.obuf = c->bout;
         proto.bytes = (double)H * ((double)L + (double)nh) * cs;
         emit_loops(c, &proto, nb, pbat, cs, cs);
-        P->align_need = (unsigned)(2 * rs);
+        need_align(c, c->bin, (unsigned)(2 * rs));
         desc_add(P, "(r2c-half n=%lld)", nl);
     } else {
         /* widen -> full complex DFT along the last dim -> copy n/2+1 bins */
@@ -1296,7 +1304,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
             p->a.aux = wk;
             p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
         }
-        if (unit) P->align_need = (unsigned)(2 * rs);
+        if (unit) need_align(c, c->bout, (unsigned)(2 * rs));
         desc_add(P, "(c2r-fused L=%lld)", L);
     } else if ((k = pick_variant(c, prec, nl, FK_ROW, FK_IO_C2R_FULL)) != 0) {
         /* one pass: Hermitian expansion on load, real part on store */
@@ -1337,7 +1345,7 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
         proto.bytes = (double)H * ((double)nh + (double)L) * cs;
         emit_loops(c, &proto, nb, pbat, cs, cs);
         if (plan_c2c_1d(c, L, 1, 1, nb, cb, FK_BUF_S0 + 6, 0, c->bout, 0, 1, 1, 0)) return -1;
-        P->align_need = (unsigned)(2 * rs);
+        need_align(c, c->bout, (unsigned)(2 * rs));
         desc_add(P, "(c2r-half n=%lld)", nl);
     } else {
         /* Hermitian expansion -> full complex inverse -> real part */
@@ -1446,7 +1454,6 @@ static int plan_real_line(ctx *c, int kind, long long n, long long is, long long
 {
     fk_problem q;
     int sbin = c->bin, sbout = c->bout, rc;
-    unsigned align = c->P->align_need;
     memset(&q, 0, sizeof(q));
     q.kind = kind;
     q.prec = c->P->prec;
@@ -1464,8 +1471,6 @@ static int plan_real_line(ctx *c, int kind, long long n, long long is, long long
     rc = kind == FK_R2C ? plan_r2c(c, &q) : plan_c2r(c, &q);
     c->bin = sbin;
     c->bout = sbout;
-    /* a real side in staging memory puts no requirement on the user's arrays */
-    if (!(kind == FK_R2C ? iptr : optr)) c->P->align_need = align;
     return rc;
 }
 
@@ -1560,7 +1565,7 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
                 for (i = 0; i < nb; ++i) quad &= (b[i].is % 4) == 0;
                 if (quad) {
                     fl |= FK_REAL_QUAD;
-                    if (P->align_need < 16) P->align_need = 16;
+                    need_align(c, ib, 16);
                 }
             }
             emit_stockham(c, k, is, os, nb, b, ib, 0, ob, 0, fl, rs, rs);
@@ -1571,7 +1576,7 @@ static int plan_r2r_1d(ctx *c, int kind, long long n, long long is, long long os
                 p->a.rmode = (unsigned)rm;
                 p->bytes = 2.0 * (double)batch_count(nb, b) * (double)n * rs;
             }
-            if (unit) P->align_need = (unsigned)(2 * rs);
+            if (unit) need_align(c, io == FK_IO_R2C_RR ? ib : ob, (unsigned)(2 * rs));
             return c->err ? -1 : 0;
         }
     }
@@ -2200,8 +2205,11 @@ int fk_execute4(fk_plan *p, void *in, void *in2, void *out, void *out2)
     void *tmp[FK_MAX_SCRATCH];
     int i, rc = 0, own = 0;
     hipStream_t st = p->stream;
-    if (((uintptr_t)in | (uintptr_t)out | (uintptr_t)in2 | (uintptr_t)out2) % p->align_need) {
-        set_err("execute: arrays not aligned as at planning (%u bytes)", p->align_need);
+    if (((uintptr_t)in | (uintptr_t)out | (uintptr_t)in2 | (uintptr_t)out2) % p->align_need ||
+        (p->align_in && ((uintptr_t)in % p->align_in)) ||
+        (p->align_out && ((uintptr_t)out % p->align_out))) {
+        set_err("execute: arrays not aligned as at planning (%u / %u / %u bytes)", p->align_need,
+                p->align_in, p->align_out);
         return (int)hipErrorInvalidValue;
     }
     if (p->split == FK_SPLIT_GENERAL && p->npass &&

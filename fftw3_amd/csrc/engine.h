@@ -90,6 +90,8 @@ typedef struct fk_plan {
     void *scratch[FK_MAX_SCRATCH];
     int scratch_busy;
     unsigned align_need;        /* required alignment of in/out (bytes) */
+    unsigned align_in, align_out; /* more for one array, where a pass reads or
+                                     writes it in pairs or quads */
     hipStream_t stream;
     int refcount;
     double adds, muls, fmas;    /* nominal operation counts */
