@@ -357,3 +357,30 @@ def test_r2r_edge_shapes(prec):
         ref = oracle.r2rn(x.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
         assert oracle.rel_l2(y, ref) < TOL[prec], (NAMES[kind], n)
         assert np.array_equal(xin, x)
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+@pytest.mark.parametrize("kind", [0, 2, 5, 9, 1, 4])
+def test_r2r_output_offset_by_one_real(prec, kind):
+    """Regression: kinds whose kernels read the input in pairs or quads
+    (R2HC, DHT, REDFT10, RODFT10) or write the output in pairs (HC2R,
+    REDFT01) must execute when the other array is only aligned to one real
+    (the alignment requirement is per array, as at planning)."""
+    n, h = 4096, 8
+    rs = np.dtype(rdt(prec)).itemsize
+    x = inputs_r(n * h, 111 + kind, prec)
+    a = fa.DeviceBuffer(x.nbytes + 16 * rs)
+    o = fa.DeviceBuffer(x.nbytes + 16 * rs)
+    a.upload(np.concatenate([x, np.zeros(16, rdt(prec))]))
+    o.upload(np.zeros(n * h + 16, rdt(prec)))
+    pi, po = a.ptr, o.ptr + rs           # output starts one real in
+    if kind in (1, 4):                   # real-output kernels: offset the input instead
+        pi, po = a.ptr + rs, o.ptr
+        a.upload(np.concatenate([np.zeros(1, rdt(prec)), x, np.zeros(15, rdt(prec))]))
+    p = fa.plan_many_r2r(prec, [n], h, pi, None, 1, n, po, None, 1, n, [kind])
+    p.execute()
+    y = o.download(rdt(prec))
+    y = y[1:1 + n * h] if kind not in (1, 4) else y[:n * h]
+    ref = oracle.r2rn(x.astype(np.float64).reshape(h, n), [kind], axes_batch=1).reshape(-1)
+    assert oracle.rel_l2(y, ref) < TOL[prec]
+    p.destroy()
