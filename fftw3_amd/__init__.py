@@ -234,6 +234,11 @@ def lib():
     L.fftw_amd_batch_stream.argtypes = [vp]
     L.fftw_amd_batch_stream.restype = vp
     L.fftw_amd_batch_destroy.argtypes = [vp]
+    L.fftw_amd_catalog_size.restype = ctypes.c_int
+    L.fftw_amd_catalog_entry.argtypes = [ip, ipp]
+    L.fftw_amd_catalog_entry.restype = ctypes.c_int
+    L.fftw_amd_force_kernel.argtypes = [ip]
+    L.fftw_amd_force_kernel.restype = ctypes.c_int
     L.fftw_amd_plan_use_mfma.argtypes = [vp]
     L.fftw_amd_mfma_probe.argtypes = [vp]
     L.fftw_amd_server_create.restype = vp
@@ -448,6 +453,32 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+IO_NAMES = ["plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r", "dct4", "r2c-full",
+            "c2r-full"]
+
+
+def catalog():
+    """The kernel catalog: one dict per entry (fftw_amd_catalog_entry)."""
+    L = lib()
+    out = []
+    info = (ctypes.c_int * 21)()
+    for i in range(L.fftw_amd_catalog_size()):
+        if L.fftw_amd_catalog_entry(i, info):
+            raise RuntimeError("catalog entry %d" % i)
+        v = list(info)
+        out.append(dict(index=i, prec="double" if v[0] == 0 else "float", L=v[1],
+                        mode="row" if v[2] == 0 else "col", io=IO_NAMES[v[3]], tpl=v[4], nl=v[5],
+                        variant=v[6], rad=v[8:8 + v[7]], lds=v[20]))
+    return out
+
+
+def force_kernel(i):
+    """Plans created from now on use catalog entry i where the planner picks
+    a kernel of its precision, length, mapping and IO kind (-1: normal)."""
+    if lib().fftw_amd_force_kernel(i):
+        raise ValueError("catalog index %d out of range" % i)
 
 
 def _iarr(v):

@@ -787,11 +787,9 @@ void X(flops)(const PLAN p, double *add, double *mul, double *fmas)
     *fmas = p ? P_(p)->fmas : 0;
 }
 
-double X(estimate_cost)(const PLAN p)
-{
-    /* HBM-bound model: bytes moved at 8 TB/s, in seconds */
-    return p ? fk_plan_bytes(P_(p)) / 8.0e12 : 0;
-}
+/* modelled seconds per execute, from the plan's passes (api/flops.c:34-40
+   returns the planner's cost estimate) */
+double X(estimate_cost)(const PLAN p) { return p ? fk_plan_estimate(P_(p)) : 0; }
 
 double X(cost)(const PLAN p) { return p ? P_(p)->pcost : 0; }
 
@@ -865,6 +863,26 @@ double fftw_amd_plan_time(void *p, void *in, void *out, int iters, double *per_p
 {
     return p ? fk_plan_time((fk_plan *)p, in, out, iters, per_pass_ms) : -1;
 }
+
+int fftw_amd_catalog_size(void) { return fk_catalog_size(); }
+int fftw_amd_catalog_entry(int i, int *info)
+{
+    const fk_kernel *k = fk_catalog_entry(i);
+    int j;
+    if (!k || !info) return -1;
+    info[0] = k->prec;
+    info[1] = k->L;
+    info[2] = k->mode;
+    info[3] = k->io;
+    info[4] = k->tpl;
+    info[5] = k->nl;
+    info[6] = k->variant;
+    info[7] = k->nrad;
+    for (j = 0; j < 12; ++j) info[8 + j] = j < k->nrad ? k->rad[j] : 0;
+    info[20] = (int)k->lds;
+    return 0;
+}
+int fftw_amd_force_kernel(int i) { return fk_force_kernel(i); }
 
 int fftw_amd_plan_use_mfma(void *p) { return fk_plan_use_mfma((fk_plan *)p); }
 int fftw_amd_mfma_probe(float *dev_out)

@@ -360,12 +360,47 @@ static int find_variants(int prec, long long L, int mode, int io, const fk_kerne
     return n;
 }
 
+/* Catalog entry forced by fk_force_kernel (NULL: none).  Wherever the
+   planner picks a kernel of the same precision, length, mapping and IO kind
+   it takes this one instead, so that every catalog entry, including the
+   variants only the autotuner would pick, can be run and checked. */
+static const fk_kernel *g_force = 0;
+
+static int forced(int prec, long long L, int mode, int io)
+{
+    const fk_kernel *f = g_force;
+    return f && f->prec == prec && f->L == L && f->mode == mode && f->io == io;
+}
+
 static const fk_kernel *pick_variant(ctx *c, int prec, long long L, int mode, int io)
 {
     const fk_kernel *vs[16];
-    int n = find_variants(prec, L, mode, io, vs, 16);
+    int n = find_variants(prec, L, mode, io, vs, 16), v;
     if (!n) return 0;
-    return vs[n > 1 ? decide(c, n, DEC_VARIANT) : 0];
+    v = n > 1 ? decide(c, n, DEC_VARIANT) : 0;
+    return forced(prec, L, mode, io) ? g_force : vs[v];
+}
+
+int fk_catalog_size(void)
+{
+    if (!g_reg) g_reg = fk_registry(&g_nreg);
+    return g_nreg;
+}
+
+const fk_kernel *fk_catalog_entry(int i)
+{
+    if (!g_reg) g_reg = fk_registry(&g_nreg);
+    return i >= 0 && i < g_nreg ? g_reg + i : 0;
+}
+
+int fk_force_kernel(int i)
+{
+    const fk_kernel *k = i < 0 ? 0 : fk_catalog_entry(i);
+    if (i >= 0 && !k) return -1;
+    pthread_mutex_lock(&g_lock);
+    g_force = k;
+    pthread_mutex_unlock(&g_lock);
+    return 0;
 }
 
 static fk_pass *new_pass(ctx *c)
@@ -578,7 +613,7 @@ static int plan_bluestein(ctx *c, long long n, long long is, long long os, int n
         return -1;
     }
     memcpy(bb, b, sizeof(fk_dim) * nb);
-    kb = find_kernel(prec, M, FK_ROW, FK_IO_BLUE);
+    kb = forced(prec, M, FK_ROW, FK_IO_BLUE) ? g_force : find_kernel(prec, M, FK_ROW, FK_IO_BLUE);
     if (kb) {
         size_t before = (size_t)c->P->npass;
         unsigned fl = (swi ? FK_SWAP_IN : 0) | (swo ? FK_SWAP_OUT : 0);
@@ -1749,22 +1784,6 @@ static int inplace_ok(const fk_problem *pb)
     return 1;
 }
 
-static void count_flops(fk_plan *P)
-{
-    double N = 1, H = 1;
-    int i;
-    for (i = 0; i < P->rank; ++i) N *= (double)P->dims[i].n;
-    for (i = 0; i < P->vrank; ++i) H *= (double)P->vdims[i].n;
-    /* nominal 5 N log2 N (2.5 for real), split as in libbench2/mflops.c:20-27 */
-    {
-        double f = (N > 1 ? 5.0 * N * log2(N) : 0) * H;
-        if (P->kind != FK_C2C) f *= 0.5;
-        P->adds = 0.6 * f;
-        P->muls = 0.4 * f;
-        P->fmas = 0;
-    }
-}
-
 /* Contiguous row-major layout of a complex tensor: dims (n of the complex
    side), then vdims outermost; writes strides into sd / sv, returns the
    element count. */
@@ -1925,7 +1944,6 @@ static fk_plan *build_plan(const fk_problem *pb0, hipStream_t st, const int *cho
     P->in2 = pb.in2;
     P->out2 = pb.out2;
     if (pb.split == FK_SPLIT_GENERAL) P->align_need = (unsigned)rsz(pb.prec);
-    count_flops(P);
     for (i = 0; i < pb.vrank; ++i)
         if (pb.vdims[i].n == 0) goto done_ok; /* howmany = 0: nothing to do */
     if (pb.split == FK_SPLIT_GENERAL) {
@@ -2020,6 +2038,7 @@ done_ok:
     }
     P->nchoice = c.ci < FK_MAX_DECIDE ? c.ci : FK_MAX_DECIDE;
     if (c.r16) fk_plan_use_mfma(P);
+    fk_count_ops(P);
     if (cout) *cout = c;
     ok = 1;
 fail:
@@ -2169,9 +2188,28 @@ void fk_plan_destroy(fk_plan *p)
     free(p);
 }
 
+/* A plan's own scratch is reused by every execute, and in async mode an
+   execute returns once its passes are queued.  Work queued on the old
+   stream may still be using the scratch, so the new stream first waits for
+   everything queued on the old one: executes of the plan stay ordered
+   across the switch. */
 void fk_plan_set_stream(fk_plan *p, hipStream_t s)
 {
-    if (p) p->stream = s;
+    hipEvent_t ev;
+    if (!p || p->stream == s) return;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
+        if (hipEventRecord(ev, p->stream) != hipSuccess ||
+            hipStreamWaitEvent(s, ev, 0) != hipSuccess) {
+            /* cannot order the streams: drain the old one instead */
+            (void)hipGetLastError();
+            (void)hipStreamSynchronize(p->stream);
+        }
+        (void)hipEventDestroy(ev);
+    } else {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(p->stream);
+    }
+    p->stream = s;
 }
 
 /* ------------------------------------------------------------------ */
@@ -2283,6 +2321,7 @@ int fk_plan_use_mfma(fk_plan *p)
         ps->a.xcd_remap = 1;
         ++n;
     }
+    if (n) fk_count_ops(p);
     return n;
 }
 
@@ -2362,6 +2401,33 @@ double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass
 /* introspection                                                       */
 
 int fk_plan_npasses(const fk_plan *p) { return p ? p->npass : 0; }
+
+/* Modelled seconds of one execute, from the plan's passes: each pass takes
+   the longer of its HBM bytes at the streaming rate and its arithmetic at
+   the vector peak of its precision, plus a launch; a pass whose contiguous
+   runs are short (strided columns of few lines) pays the penalty of
+   seg_pen().  This is the planner-side estimate that fftw_estimate_cost
+   reports (the role of kernel/planner.c:426-442's cost model). */
+double fk_plan_estimate(const fk_plan *p)
+{
+    const double bw = 5.2e12, launch = 3e-6;
+    const double peak = p && p->prec ? 157.3e12 : 78.6e12;
+    double t = 0;
+    int i;
+    if (!p) return 0;
+    for (i = 0; i < p->npass; ++i) {
+        const fk_pass *ps = p->pass + i;
+        double tb = ps->bytes / bw, tf = 0;
+        if (ps->type == FK_PASS_STOCKHAM) {
+            const fk_kernel *k = ps->k;
+            double fl = 5.0 * k->L * log2((double)k->L) * (double)ps->a.nlines;
+            if (k->mode == FK_COL) tb *= 1.0 + seg_pen((double)k->nl * csz(k->prec));
+            tf = fl / peak;
+        }
+        t += (tb > tf ? tb : tf) + launch;
+    }
+    return t;
+}
 
 double fk_plan_bytes(const fk_plan *p)
 {

@@ -94,7 +94,7 @@ typedef struct fk_plan {
                                      writes it in pairs or quads */
     hipStream_t stream;
     int refcount;
-    double adds, muls, fmas;    /* nominal operation counts */
+    double adds, muls, fmas;    /* operation counts of the passes (opcount.c) */
     double pcost;               /* measured seconds per execute (MEASURE) */
     char desc[2048];            /* plan description for print_plan */
     int choice[32];             /* planner decisions (wisdom payload) */
@@ -155,6 +155,14 @@ char *fk_plan_sprint(const fk_plan *p);
 int fk_plan_npasses(const fk_plan *p);
 int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len);
 double fk_plan_bytes(const fk_plan *p);
+double fk_plan_estimate(const fk_plan *p);
+/* kernel catalog introspection and the planner's forced-entry test hook */
+int fk_catalog_size(void);
+const fk_kernel *fk_catalog_entry(int i);
+int fk_force_kernel(int i);
+/* operation counts of the plan's passes (opcount.c) */
+void fk_count_ops(fk_plan *p);
+void fk_dft_ops(int n, double *add, double *mul, double *fma);
 double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass_ms);
 
 /* batched uneven sizes (batch.c) */

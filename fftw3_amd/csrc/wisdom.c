@@ -16,12 +16,16 @@
  *     ...)
  */
 #include <pthread.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "engine.h"
 
 #define MAXD 32
+/* largest decision value accepted at import: a decision indexes the
+   alternatives of one planner choice (at most 16 kernel variants) */
+#define MAXC 63
 
 typedef struct {
     unsigned long long h;
@@ -117,25 +121,52 @@ int fk_wisdom_lookup(unsigned long long h, int *level, int *c, int *n)
     return found;
 }
 
+/* appends to a growable string; returns 0 when memory runs out */
+typedef struct {
+    char *s;
+    size_t len, cap;
+} sbuf;
+
+static int sb_printf(sbuf *b, const char *fmt, ...)
+{
+    va_list ap;
+    int need;
+    va_start(ap, fmt);
+    need = vsnprintf(0, 0, fmt, ap);
+    va_end(ap);
+    if (need < 0) return 0;
+    if (b->len + (size_t)need + 1 > b->cap) {
+        size_t cap = 2 * b->cap + (size_t)need + 1;
+        char *t = (char *)realloc(b->s, cap);
+        if (!t) return 0;
+        b->s = t;
+        b->cap = cap;
+    }
+    va_start(ap, fmt);
+    vsnprintf(b->s + b->len, b->cap - b->len, fmt, ap);
+    va_end(ap);
+    b->len += (size_t)need;
+    return 1;
+}
+
 char *fk_wisdom_export(void)
 {
-    size_t cap = 256 + 64 * 16, len;
-    char *s;
-    int i, j;
+    sbuf b = {0, 0, 0};
+    int i, j, ok;
     pthread_mutex_lock(&g_wlock);
-    cap += (size_t)g_n * (48 + 8 * MAXD);
-    s = (char *)malloc(cap);
-    len = (size_t)snprintf(s, cap, "(fftw3-amd-gfx950-wisdom #x%016llx\n", catalog_hash());
-    for (i = 0; i < g_n; ++i) {
-        len += (size_t)snprintf(s + len, cap - len, "  (#x%016llx %d %d", g_tab[i].h,
-                                g_tab[i].level, g_tab[i].n);
-        for (j = 0; j < g_tab[i].n; ++j)
-            len += (size_t)snprintf(s + len, cap - len, " %d", g_tab[i].c[j]);
-        len += (size_t)snprintf(s + len, cap - len, ")\n");
+    ok = sb_printf(&b, "(fftw3-amd-gfx950-wisdom #x%016llx\n", catalog_hash());
+    for (i = 0; ok && i < g_n; ++i) {
+        ok = sb_printf(&b, "  (#x%016llx %d %d", g_tab[i].h, g_tab[i].level, g_tab[i].n);
+        for (j = 0; ok && j < g_tab[i].n; ++j) ok = sb_printf(&b, " %d", g_tab[i].c[j]);
+        if (ok) ok = sb_printf(&b, ")\n");
     }
-    snprintf(s + len, cap - len, ")\n");
+    if (ok) ok = sb_printf(&b, ")\n");
     pthread_mutex_unlock(&g_wlock);
-    return s;
+    if (!ok) {
+        free(b.s);
+        return 0;
+    }
+    return b.s;
 }
 
 static const char *skip_ws(const char *p)
@@ -168,7 +199,8 @@ int fk_wisdom_import(const char *s)
         p += used;
         if (e.n < 0 || e.n > MAXD || e.level < 0 || e.level > 3) goto bad;
         for (i = 0; i < e.n; ++i) {
-            if (sscanf(p, " %d%n", &e.c[i], &used) != 1 || e.c[i] < 0) goto bad;
+            if (sscanf(p, " %d%n", &e.c[i], &used) != 1 || e.c[i] < 0 || e.c[i] > MAXC)
+                goto bad;
             p += used;
         }
         p = skip_ws(p);

@@ -207,8 +207,10 @@ def catalog():
                     g = geometry(prec, L, rads, mode, "DCT4")
                     if g:
                         entries.append((prec, L, rads, mode, "DCT4", g))
-            # fused Bluestein on power-of-two convolution lengths
-            if L & (L - 1) == 0 and L >= 4:
+            # fused Bluestein on power-of-two convolution lengths; below 64
+            # every length n with 2n - 1 <= L is 13-smooth and never takes
+            # Bluestein
+            if L & (L - 1) == 0 and L >= 64:
                 g = geometry(prec, L, rads, "ROW", "BLUE")
                 if g:
                     entries.append((prec, L, rads, "ROW", "BLUE", g))

@@ -23,13 +23,17 @@ int fftw_amd_init(void);
 const char *fftw_amd_last_error(void);
 
 /* Stream used by plans created from now on (hipStream_t, 0 = null stream),
-   and the stream of one existing plan. */
+   and the stream of one existing plan.  Switching a plan's stream makes the
+   new stream wait for the work already queued on the old one, so async
+   executes of one plan, which share its scratch, never overlap. */
 void fftw_amd_set_stream(void *hip_stream);
 void *fftw_amd_get_stream(void);
 void fftw_amd_plan_set_stream(void *plan, void *hip_stream);
 
 /* 0 (default): execute returns after the transform completed, as FFTW's
-   execute does.  1: execute only enqueues work on the plan's stream. */
+   execute does.  1: execute only enqueues work on the plan's stream; the
+   plan's scratch is then reused by the next execute, which is safe because
+   every execute of a plan goes to the plan's stream (in queue order). */
 void fftw_amd_set_async(int on);
 
 /* Device memory helpers; return 0 on success. */
@@ -59,6 +63,23 @@ double fftw_amd_plan_pass_bytes(const void *plan, int i);
    fftw_amd_plan_npasses entries) and returns the mean milliseconds of a
    whole execute.  Negative on error. */
 double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *per_pass_ms);
+
+/* Kernel catalog (the compiled Stockham kernels the planner chooses from).
+   fftw_amd_catalog_entry fills info[21]: precision (0 double, 1 float),
+   length L, mapping (0 row, 1 column), IO kind (0 plain, 1 transposed
+   store, 2 r2c, 3 c2r, 4 Bluestein, 5 r2c with a real-to-real kind, 6 c2r
+   with a real-to-real kind, 7 DCT-IV, 8 full-length r2c, 9 full-length
+   c2r), threads per line, lines per workgroup, variant (0 = the default
+   geometry), number of radices, the radices (12 slots), dynamic LDS bytes.
+   Returns 0, or -1 for an index out of range. */
+int fftw_amd_catalog_size(void);
+int fftw_amd_catalog_entry(int i, int *info);
+/* Test hook: plans created after this call use catalog entry i wherever
+   the planner would pick a kernel of the same precision, length, mapping
+   and IO kind (whichever variant it would have chosen); -1 restores normal
+   planning.  Lets every catalog entry, including the variants only the
+   autotuner picks, be run and checked.  Returns 0, or -1 for a bad index. */
+int fftw_amd_force_kernel(int i);
 
 /* Split-precision matrix-core variant (FP32 plans only): every pass of
    length 16, 256 or 4096 with radix-16 stages runs its butterflies as FP16

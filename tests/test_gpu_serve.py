@@ -108,7 +108,7 @@ def test_admission_by_budget(eng):
         fa.lib().fftw_amd_memcpy_dtoh(y.ctypes.data, ptr, y.nbytes)
         ref = oracle_ref("c2c", (n,), h, xs[k])
         assert oracle.rel_l2(y, ref) <= TOL["float"]
-        srv.release(k)
+        assert srv.release(ids[k]) == 0
     # larger than the whole budget: rejected at submission
     big = fa.DeviceBuffer(1 << 22)
     assert srv.submit(fa.Server.request("c2c", "float", (1 << 18,), 1, big.ptr)) == -1

@@ -73,6 +73,28 @@ def test_file_roundtrip(tmp_path):
     L.fftw_forget_wisdom()
 
 
+def test_large_entries_export_exactly():
+    """Many entries with the most decisions and the largest accepted values
+    export completely; decision values beyond the accepted bound (and
+    beyond int range) are rejected at import, atomically."""
+    L = fa.lib()
+    L.fftw_forget_wisdom()
+    tag = catalog_tag(export())
+    ent = ["  (#x%016x 3 32 %s)" % (0x1000 + i, " ".join(["63"] * 32)) for i in range(200)]
+    s = "(fftw3-amd-gfx950-wisdom %s\n%s\n)\n" % (tag, "\n".join(ent))
+    assert L.fftw_import_wisdom_from_string(s.encode()) == 1
+    out = export()
+    assert out.count("(#x") == 200
+    for e in ent:
+        assert e.strip() in out
+    assert out.rstrip().endswith(")")
+    for big in ("64", "2147483647", "99999999999"):
+        bad = "(fftw3-amd-gfx950-wisdom %s\n  (#x0000000000000077 1 2 0 %s)\n)\n" % (tag, big)
+        assert L.fftw_import_wisdom_from_string(bad.encode()) == 0
+    assert "#x0000000000000077" not in export()
+    L.fftw_forget_wisdom()
+
+
 def _run(prec, n, howmany, flags, seed=5):
     from gpu_util import c2c_many
     return c2c_many(prec, [n], howmany, -1, seed=seed, flags=flags)
