@@ -211,6 +211,8 @@ def lib():
         getattr(L, n).argtypes = [vp, vp, ctypes.c_size_t]
         getattr(L, n).restype = ctypes.c_int
     L.fftw_amd_memset.argtypes = [vp, ip, ctypes.c_size_t]
+    L.fftw_amd_stream_copy.argtypes = [vp, vp, ctypes.c_size_t, ip, ip, vp]
+    L.fftw_amd_stream_copy.restype = ctypes.c_int
     L.fftw_amd_synchronize.restype = ctypes.c_int
     L.fftw_amd_mem_free.restype = ctypes.c_size_t
     for n in ("fftw_amd_fill_uniform", "fftw_amd_fill_gauss"):

@@ -827,6 +827,10 @@ int fftw_amd_memcpy_dtod(void *d, const void *s, size_t b)
     return (int)hipMemcpy(d, s, b, hipMemcpyDeviceToDevice);
 }
 int fftw_amd_memset(void *d, int v, size_t b) { return (int)hipMemset(d, v, b); }
+int fftw_amd_stream_copy(void *d, const void *s, size_t b, int mode, int ctas, void *stream)
+{
+    return fkm_stream_copy(d, s, (long long)b, mode, ctas, (hipStream_t)stream);
+}
 int fftw_amd_synchronize(void) { return (int)hipDeviceSynchronize(); }
 
 size_t fftw_amd_mem_free(void)

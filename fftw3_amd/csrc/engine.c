@@ -2456,9 +2456,9 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
             snprintf(rad + strlen(rad), sizeof(rad) - strlen(rad), "%s%d", j ? "x" : "",
                      ps->k->rad[j]);
         return snprintf(buf, len,
-                        "stockham %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
+                        "%s %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
                         "is=%lld os=%lld flags=%u",
-                        ps->k->prec ? "f32" : "f64", ps->k->L, rad,
+                        ps->k->variant >= 100 ? "mfma16" : "stockham", ps->k->prec ? "f32" : "f64", ps->k->L, rad,
                         ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
                         ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
                         ps->a.os, ps->a.flags);

@@ -203,6 +203,7 @@ int fkm_gen_tw2(double *lo, double *hi, long long N, long long b, hipStream_t st
 int fkm_gen_line_tw(void *out, int prec, long long n, long long count, hipStream_t st);
 int fkm_gen_chirp(void *out, int prec, long long n, hipStream_t st);
 int fkm_gen_blue_b(void *out, long long n, long long M, hipStream_t st);
+int fkm_stream_copy(void *dst, const void *src, long long bytes, int mode, int ctas, hipStream_t st);
 int fkm_scale_round(void *out, int prec, const void *in, long long count, double scale,
                     hipStream_t st);
 int fkm_fill(void *out, int prec, long long count, unsigned long long seed,

@@ -412,6 +412,44 @@ __global__ void __launch_bounds__(EW_BS) k_ew(fk_ew_args a, int lpb, long long s
     }
 }
 
+/* Streaming copy, the practical HBM ceiling the transform passes are
+   measured against: 16-byte loads and stores, U of them in flight per
+   thread, each workgroup on a contiguous chunk of U*256*16 bytes per
+   iteration; NT selects non-temporal accesses (no cache allocation).
+   n16 counts 16-byte units; a byte tail is copied by the last thread. */
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_stream_copy(const u32x4 *__restrict__ src,
+                                                     u32x4 *__restrict__ dst, long long n16)
+{
+    const long long step = (long long)gridDim.x * 256 * U;
+    for (long long base = (long long)blockIdx.x * 256 * U + threadIdx.x; base < n16; base += step) {
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = base + (long long)u * 256;
+            if (i < n16) {
+                if constexpr (NT) v[u] = __builtin_nontemporal_load(src + i);
+                else v[u] = src[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = base + (long long)u * 256;
+            if (i < n16) {
+                if constexpr (NT) __builtin_nontemporal_store(v[u], dst + i);
+                else dst[i] = v[u];
+            }
+        }
+    }
+}
+
+__global__ void k_copy_tail(const unsigned char *src, unsigned char *dst, long long n)
+{
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 } // namespace
@@ -503,6 +541,31 @@ int fkm_fill(void *out, int prec, long long count, unsigned long long seed,
             hipLaunchKernelGGL(k_fill_uniform<float>, dim3(nb), dim3(256), 0, st, (float *)out,
                                count, seed, stream);
     }
+    return (int)hipGetLastError();
+}
+
+/* Streaming copy of `bytes` (16-byte aligned pointers); mode 0: plain
+   16-byte accesses, 1: non-temporal.  The grid covers every CU several
+   times over (ctas_per_cu workgroups per CU) and loops over the rest. */
+int fkm_stream_copy(void *dst, const void *src, long long bytes, int mode, int ctas, hipStream_t st)
+{
+    const long long n16 = bytes / 16, tail = bytes - n16 * 16;
+    if (((uintptr_t)dst | (uintptr_t)src) % 16) return (int)hipErrorInvalidValue;
+    if (ctas <= 0) ctas = 2048;
+    long long need = (n16 + 256 * 4 - 1) / (256 * 4);
+    unsigned grid = (unsigned)(need < ctas ? (need > 0 ? need : 1) : ctas);
+    if (n16 > 0) {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_stream_copy<4, true>), dim3(grid), dim3(256), 0, st,
+                               (const u32x4 *)src, (u32x4 *)dst, n16);
+        else
+            hipLaunchKernelGGL((k_stream_copy<4, false>), dim3(grid), dim3(256), 0, st,
+                               (const u32x4 *)src, (u32x4 *)dst, n16);
+    }
+    if (tail)
+        hipLaunchKernelGGL(k_copy_tail, dim3(1), dim3(64), 0, st,
+                           (const unsigned char *)src + n16 * 16, (unsigned char *)dst + n16 * 16,
+                           tail);
     return (int)hipGetLastError();
 }
 

@@ -41,6 +41,14 @@ int fftw_amd_memcpy_htod(void *dst, const void *src, size_t bytes);
 int fftw_amd_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int fftw_amd_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int fftw_amd_memset(void *dst, int value, size_t bytes);
+/* Streaming device copy (hand-written kernel: 16-byte accesses, four in
+   flight per thread, `ctas` workgroups of 256 threads looping over the
+   array, 0 = 2048); mode 0 plain, 1 non-temporal.  Enqueued on `stream`
+   (hipStream_t, 0 = null stream), not synchronized.  16-byte aligned
+   pointers; returns 0 or a HIP error code.  The benchmark's practical HBM
+   ceiling. */
+int fftw_amd_stream_copy(void *dst, const void *src, size_t bytes, int mode, int ctas,
+                         void *hip_stream);
 int fftw_amd_synchronize(void);
 size_t fftw_amd_mem_free(void);
 

@@ -206,6 +206,24 @@ def dft_bins(x, bins, sign=-1):
     return out
 
 
+def dft_bins_nd(x, bins, sign=-1):
+    """Sampled bins of the DFT over all axes of a C-contiguous array
+    (complex, or real with zero imaginary parts); bins: (nbins, ndim)."""
+    real = not np.iscomplexobj(x)
+    x = np.ascontiguousarray(x, dtype=np.float64 if real else np.complex128)
+    b = np.ascontiguousarray(np.asarray(bins).reshape(-1, x.ndim), dtype=np.int64)
+    n = (ctypes.c_long * x.ndim)(*x.shape)
+    out = np.empty(b.shape[0], dtype=np.complex128)
+    f = lib().orc_dft_bins_nd
+    f.restype = None
+    f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_long), ctypes.c_int,
+                  ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.POINTER(ctypes.c_long),
+                  ctypes.c_long, ctypes.POINTER(ctypes.c_double)]
+    f(x.ndim, n, sign, _dp(x), int(real), b.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+      b.shape[0], _dp(out))
+    return out
+
+
 def cexp(m, n, sign=-1):
     out = np.empty(2)
     lib().orc_cexp(m, n, sign, _dp(out))

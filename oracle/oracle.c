@@ -491,6 +491,82 @@ void orc_dft_bins(long n, int sign, const double *in, const long *bins,
     tbl2_free(&t);
 }
 
+/* Sampled bins of a rank-r DFT of a row-major array (n[0] slowest): bin
+   (k_0, ..., k_{r-1}) = sum_j x_j prod_d w_{n_d}^{j_d k_d}, summed one
+   dimension at a time from the fastest (separability), in long double with
+   exactly reduced exponents.  `real` says the input holds reals (imaginary
+   parts zero); bins holds nbins rank-tuples. */
+void orc_dft_bins_nd(int rank, const long *n, int sign, const double *in, int real,
+                     const long *bins, long nbins, double *out)
+{
+    long b;
+    tbl2 t[8];
+    int d;
+    if (rank < 1 || rank > 8) return;
+    for (d = 0; d < rank; ++d) tbl2_make(&t[d], n[d], sign);
+    for (b = 0; b < nbins; ++b) {
+        const long *k = bins + (long)rank * b;
+        long outer = 1, m, i;
+        long double *acc, *nxt;
+        for (d = 0; d < rank - 1; ++d) outer *= n[d];
+        acc = (long double *)malloc(sizeof(long double) * 2 * outer);
+        /* fastest dimension, straight from the input */
+#pragma omp parallel for schedule(static)
+        for (i = 0; i < outer; ++i) {
+            const long nl = n[rank - 1];
+            long kk = k[rank - 1] % nl, e = 0, j;
+            long double re = 0, im = 0;
+            if (kk < 0) kk += nl;
+            for (j = 0; j < nl; ++j) {
+                long double xr, xi, wr, wi;
+                if (real) {
+                    xr = in[i * nl + j];
+                    xi = 0;
+                } else {
+                    xr = in[2 * (i * nl + j)];
+                    xi = in[2 * (i * nl + j) + 1];
+                }
+                tbl2_get(&t[rank - 1], e, &wr, &wi);
+                re += xr * wr - xi * wi;
+                im += xr * wi + xi * wr;
+                e += kk;
+                if (e >= nl) e -= nl;
+            }
+            acc[2 * i] = re;
+            acc[2 * i + 1] = im;
+        }
+        /* the remaining dimensions on the partial sums */
+        for (d = rank - 2; d >= 0; --d) {
+            const long nd = n[d];
+            long kk = k[d] % nd;
+            if (kk < 0) kk += nd;
+            m = outer / nd;
+            nxt = (long double *)malloc(sizeof(long double) * 2 * (m ? m : 1));
+            for (i = 0; i < m; ++i) {
+                long e = 0, j;
+                long double re = 0, im = 0;
+                for (j = 0; j < nd; ++j) {
+                    long double xr = acc[2 * (i * nd + j)], xi = acc[2 * (i * nd + j) + 1], wr, wi;
+                    tbl2_get(&t[d], e, &wr, &wi);
+                    re += xr * wr - xi * wi;
+                    im += xr * wi + xi * wr;
+                    e += kk;
+                    if (e >= nd) e -= nd;
+                }
+                nxt[2 * i] = re;
+                nxt[2 * i + 1] = im;
+            }
+            free(acc);
+            acc = nxt;
+            outer = m;
+        }
+        out[2 * b] = (double)acc[0];
+        out[2 * b + 1] = (double)acc[1];
+        free(acc);
+    }
+    for (d = 0; d < rank; ++d) tbl2_free(&t[d]);
+}
+
 /* ---- counter-based inputs ---- */
 
 uint64_t orc_splitmix64(uint64_t z)

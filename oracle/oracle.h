@@ -79,6 +79,10 @@ void orc_dft_bins(long n, int sign, const double *in, const long *bins,
 /* Twiddle exp(sign*2*pi*i*m/n) in long double, octant-reduced; written as
    two doubles (the rounded value). */
 void orc_cexp(long m, long n, int sign, double *out2);
+/* sampled bins (rank-tuples in `bins`) of the rank-r DFT of a row-major
+   complex (real = 0) or real (real = 1) array, long double sums */
+void orc_dft_bins_nd(int rank, const long *n, int sign, const double *in, int real,
+                     const long *bins, long nbins, double *out);
 
 /* Counter-based inputs: x[i] = 2*u - 1, u = top 53 bits of
    splitmix64(seed ^ (stream << 40) ^ i) / 2^53.  Same stream on the GPU. */

@@ -199,3 +199,22 @@ def test_planted_mistakes_are_caught(orc):
     X = orc.rfftn(np.linspace(-1, 1, n))
     bad = np.conj(X)
     assert orc.rel_l2(orc.irfftn(bad, n) / n, np.linspace(-1, 1, n)) > 0.1
+
+
+@pytest.mark.parametrize("shape", [(6, 10, 14), (16, 9), (7, 5, 4, 3), (1000,)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_dft_bins_nd_against_numpy(orc, shape, sign):
+    """Sampled multi-D bins (the full-size checks use them) against
+    numpy.fft.fftn, complex and real inputs, negative bin indices."""
+    rng = np.random.default_rng(len(shape) * 7 + sign)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    F = np.fft.fftn(x) if sign < 0 else np.fft.ifftn(x) * x.size
+    bins = np.stack([rng.integers(-n, n, 6) for n in shape], axis=1)
+    got = orc.dft_bins_nd(x, bins, sign)
+    want = np.array([F[tuple(b)] for b in bins])
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(F))
+    xr = rng.standard_normal(shape)
+    Fr = np.fft.fftn(xr) if sign < 0 else np.fft.ifftn(xr) * xr.size
+    got = orc.dft_bins_nd(xr, bins, sign)
+    want = np.array([Fr[tuple(b)] for b in bins])
+    assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(Fr))
