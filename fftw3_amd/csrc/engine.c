@@ -256,13 +256,25 @@ fail:
 /* ------------------------------------------------------------------ */
 /* kernel catalog queries                                              */
 
+/* Catalog entries the planner never picks: the FP64 column-mapped c2r
+   kernels with a fused real-to-real kind and one line per workgroup
+   (L = 5120, 6144, 8192).  An execute through the 8192-point one aborted
+   on the device and the cause is not established; strided HC2R / REDFT01 /
+   RODFT01 lines of those lengths take the row-mapped kernel instead. */
+static int kernel_disabled(const fk_kernel *k)
+{
+    return k->prec == 0 && k->mode == FK_COL && k->io == FK_IO_C2R_RR && k->nl == 1;
+}
+
 static const fk_kernel *find_kernel(int prec, long long L, int mode, int io)
 {
     int i;
     if (L > (1 << 20)) return 0;
     for (i = 0; i < g_nreg; ++i) {
         const fk_kernel *k = g_reg + i;
-        if (k->prec == prec && k->L == L && k->mode == mode && k->io == io) return k;
+        if (k->prec == prec && k->L == L && k->mode == mode && k->io == io &&
+            !kernel_disabled(k))
+            return k;
     }
     return 0;
 }
@@ -277,7 +289,8 @@ static const fk_kernel *find_kernel_r16(int prec, long long L, int mode, int io)
     for (i = 0; i < g_nreg; ++i) {
         const fk_kernel *k = g_reg + i;
         int all16 = k->nrad > 0;
-        if (k->prec != prec || k->L != L || k->mode != mode || k->io != io) continue;
+        if (k->prec != prec || k->L != L || k->mode != mode || k->io != io || kernel_disabled(k))
+            continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (all16) return k;
     }
@@ -354,7 +367,7 @@ static int find_variants(int prec, long long L, int mode, int io, const fk_kerne
                 break;
             }
         }
-        if (!hit) break;
+        if (!hit || kernel_disabled(hit)) break;
         out[n++] = hit;
     }
     return n;
@@ -397,6 +410,7 @@ int fk_force_kernel(int i)
 {
     const fk_kernel *k = i < 0 ? 0 : fk_catalog_entry(i);
     if (i >= 0 && !k) return -1;
+    if (k && kernel_disabled(k)) return -2;
     pthread_mutex_lock(&g_lock);
     g_force = k;
     pthread_mutex_unlock(&g_lock);
@@ -840,8 +854,10 @@ static int plan_c2c_1d(ctx *c, long long n, long long is, long long os, int nb_,
         if (c->r16 && is_pow16(n)) {
             /* matrix-core variant: radix-16 passes only (16, 256, 4096) */
             ncand = 0;
-            if (n % 4096 == 0 && is_pow16(n / 4096)) cands[ncand++] = 4096;
+            /* 256 first: its column passes hold 16 lines per tile (128-byte
+               runs), where a 4096-point column pass holds 2 */
             if (n % 256 == 0 && is_pow16(n / 256)) cands[ncand++] = 256;
+            if (n % 4096 == 0 && is_pow16(n / 4096)) cands[ncand++] = 4096;
             if (is_pow16(n / 16)) cands[ncand++] = 16;
         }
         fk_dim ab[FK_MAXRANK * 2 + 1], bb[FK_MAXRANK * 2 + 1];
@@ -2093,19 +2109,33 @@ static double now_s(void)
     return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
-/* mean ms of one execute on the planning arrays (FFTW_MEASURE may
-   overwrite them, doc/reference.texi:448-450) */
-static double time_plan(fk_plan *P, int reps)
+/* Candidate timing, the method of kernel/timer.c:142-181: `iter`
+   back-to-back executes between two device events, the minimum over
+   TUNE_REPEAT such runs, with `iter` doubled from 1 until that minimum is
+   at least TUNE_MIN_MS of device time; one candidate stops repeating after
+   TUNE_LIMIT_S of wall time (FFTW_TIME_LIMIT).  The executes run on the
+   planning arrays (FFTW_MEASURE may overwrite them,
+   doc/reference.texi:448-450).  Returns ms per execute. */
+#define TUNE_REPEAT 8
+#define TUNE_MIN_MS 1.0
+#define TUNE_LIMIT_S 2.0
+static double fk_plan_time_run(fk_plan *p, int iter);
+static double time_plan(fk_plan *P)
 {
-    double t, best = 1e30;
-    int r;
-    if (fk_plan_time(P, P->in, P->out, 1, 0) < 0) return 1e30; /* warm-up */
-    for (r = 0; r < reps; ++r) {
-        t = fk_plan_time(P, P->in, P->out, 2, 0);
-        if (t < 0) return 1e30;
-        if (t < best) best = t;
+    double t0 = now_s(), tmin = 1e30;
+    int iter, r;
+    if (fk_plan_time_run(P, 1) < 0) return 1e30; /* warm-up */
+    for (iter = 1; iter <= (1 << 16); iter *= 2) {
+        tmin = 1e30;
+        for (r = 0; r < TUNE_REPEAT; ++r) {
+            double t = fk_plan_time_run(P, iter);
+            if (t < 0) return 1e30;
+            if (t < tmin) tmin = t;
+            if (now_s() - t0 > TUNE_LIMIT_S) break;
+        }
+        if (tmin >= TUNE_MIN_MS || now_s() - t0 > TUNE_LIMIT_S) break;
     }
-    return best;
+    return tmin / iter;
 }
 
 /*
@@ -2136,7 +2166,7 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
     P = build_plan(pb, st, 0, 0, &cb);
     if (!P || level == 0 || !P->in || !P->out || P->npass == 0) return P;
     t0 = now_s();
-    best = time_plan(P, level >= 3 ? 4 : 2);
+    best = time_plan(P);
     ncur = P->nchoice;
     memcpy(cur, P->choice, sizeof(int) * ncur);
     for (i = 0; i < ncur && i < FK_MAX_DECIDE; ++i) {
@@ -2150,7 +2180,7 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
             trial[i] = a;
             Q = build_plan(pb, st, trial, ncur, &cq);
             if (!Q) continue;
-            t = time_plan(Q, level >= 3 ? 4 : 2);
+            t = time_plan(Q);
             if (t < best * 0.98) {
                 fk_plan_destroy(P);
                 P = Q;
@@ -2315,9 +2345,10 @@ int fk_plan_use_mfma(fk_plan *p)
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (!all16 || (k->L != 16 && k->L != 256 && k->L != 4096)) continue;
         ps->k = &mk[k->L == 16 ? 0 : k->L == 256 ? 1 : 2];
-        ps->block = (unsigned)ps->k->tpl;
+        /* tiles of nl lines, tpl * nl threads, static LDS only */
+        ps->block = (unsigned)(ps->k->tpl * ps->k->nl);
         ps->lds = 0;
-        ps->grid = (unsigned)ps->a.nlines;
+        ps->grid = (unsigned)((ps->a.nlines + ps->k->nl - 1) / ps->k->nl);
         ps->a.xcd_remap = 1;
         ++n;
     }
@@ -2397,6 +2428,36 @@ double fk_plan_time(fk_plan *p, void *in, void *out, int iters, double *per_pass
     return total / iters;
 }
 
+/* Device ms of `iter` executes queued back to back on the planning arrays
+   and the plan's stream, between two events (no host gap inside). */
+static double fk_plan_time_run(fk_plan *p, int iter)
+{
+    void *buf[FK_NBUF];
+    hipEvent_t e0 = 0, e1 = 0;
+    hipStream_t st = p->stream;
+    float ms = -1.f;
+    int i, it, rc = 0;
+    buf[FK_BUF_IN] = p->in;
+    buf[FK_BUF_OUT] = p->out;
+    buf[FK_BUF_IN2] = p->in2;
+    buf[FK_BUF_OUT2] = p->out2;
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = -1;
+    if (!rc) rc = (int)hipEventRecord(e0, st);
+    for (it = 0; it < iter && !rc; ++it)
+        for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
+    if (!rc) rc = (int)hipEventRecord(e1, st);
+    if (!rc) rc = (int)hipEventSynchronize(e1);
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = -1;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) {
+        set_err("timing failed");
+        return -1;
+    }
+    return ms;
+}
+
 /* ------------------------------------------------------------------ */
 /* introspection                                                       */
 
@@ -2437,6 +2498,14 @@ double fk_plan_bytes(const fk_plan *p)
     return b;
 }
 
+static const char *buf_name(int b)
+{
+    static const char *n[] = {"in", "out", "in2", "out2"};
+    static const char *s[] = {"s0", "s1", "s2", "s3", "s4", "s5", "s6", "s7", "s8", "s9", "s10"};
+    if (b >= 0 && b < FK_BUF_S0) return n[b];
+    return b >= FK_BUF_S0 && b < FK_NBUF ? s[b - FK_BUF_S0] : "?";
+}
+
 static const char *io_name(int io)
 {
     static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r",
@@ -2457,14 +2526,14 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
                      ps->k->rad[j]);
         return snprintf(buf, len,
                         "%s %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
-                        "is=%lld os=%lld flags=%u",
+                        "is=%lld os=%lld flags=%u src=%s dst=%s",
                         ps->k->variant >= 100 ? "mfma16" : "stockham", ps->k->prec ? "f32" : "f64", ps->k->L, rad,
                         ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
                         ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
-                        ps->a.os, ps->a.flags);
+                        ps->a.os, ps->a.flags, buf_name(ps->ibuf), buf_name(ps->obuf));
     }
-    return snprintf(buf, len, "elementwise op=%d n=%lld lines=%lld", ps->e.op, ps->e.n,
-                    ps->e.nlines);
+    return snprintf(buf, len, "elementwise op=%d n=%lld lines=%lld src=%s dst=%s", ps->e.op,
+                    ps->e.n, ps->e.nlines, buf_name(ps->ibuf), buf_name(ps->obuf));
 }
 
 char *fk_plan_sprint(const fk_plan *p)

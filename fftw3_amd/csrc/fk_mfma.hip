@@ -67,31 +67,71 @@ __device__ __forceinline__ void split(float x, _Float16 &hi, _Float16 &lo)
     lo = (_Float16)(x - (float)hi);
 }
 
-template <int S>
-__global__ void __launch_bounds__(64 * ((S == 3) ? 8 : 1)) k_mfma(fk_args a)
+/*
+ * NL lines of length L = 16^S per workgroup of NT threads, staged through
+ * LDS: a coalesced load of the whole tile (threads along the line for unit
+ * element stride, across lines otherwise), S radix-16 stages on the matrix
+ * cores with the tile in LDS, and a coalesced store of the tile with the
+ * output post-op (four-step twiddle, scale, backward swap).  Each wave runs
+ * groups of 32 butterflies (one MFMA column each) of the tile's
+ * NL * L/16 butterflies per stage.
+ */
+template <int S, int NL, int NT>
+__global__ void __launch_bounds__(NT) k_mfma(fk_args a)
 {
     constexpr int L = S == 1 ? 16 : S == 2 ? 256 : 4096;
-    constexpr int NB = L / 16;          /* butterflies per stage */
-    constexpr int W = S == 3 ? 8 : 1;   /* waves per workgroup */
-    __shared__ cx<float> lds[L + L / 16 + 2];
-    auto idx = [](int j) { return j + (j >> 4); };
+    constexpr int NB = L / 16;                /* butterflies per line and stage */
+    constexpr int W = NT / 64;                /* waves */
+    constexpr int NG = NL * NB / 32;          /* groups of 32 butterflies */
+    constexpr int GPW = (NG + W - 1) / W;     /* groups per wave */
+    constexpr int LP = L + L / 16 + 2;        /* LDS pitch of a line */
+    static_assert((NL * NB) % 32 == 0, "tile must hold whole MFMA groups");
+    static_assert(NT % NL == 0 && NT % 64 == 0, "thread layout");
+    constexpr long long NONE = -0x7fffffffffffffffLL - 1;
+    __shared__ cx<float> lds[NL * LP];
+    __shared__ long long lin[NL], lout[NL], li0[NL];
+    auto idx = [](int lt, int j) { return lt * LP + j + (j >> 4); };
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int n = lane & 31, h = lane >> 5;
-    /* XCD-contiguous line order: neighbouring lines (which share cache
+    /* XCD-contiguous tile order: neighbouring tiles (which share cache
        lines when the pass is strided) run on one XCD and its L2 */
-    const long long line = fk_remap_line(a.xcd_remap);
-    const bool act = line < a.nlines;
-    long long q = act ? line : 0;
-    const long long i0 = q % a.nb[0];
-    q /= a.nb[0];
-    const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
-    const long long bin = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
-    const long long bout = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
-    const cx<float> *in = reinterpret_cast<const cx<float> *>(a.in) + bin;
+    const long long tile = fk_remap_line(a.xcd_remap);
+    if (tid < NL) {
+        const long long line = tile * NL + tid;
+        const bool act = line < a.nlines;
+        long long q = act ? line : 0;
+        const long long i0 = q % a.nb[0];
+        q /= a.nb[0];
+        const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
+        /* offsets may be negative (negative strides): inactive lines are
+           marked with the one value no offset takes */
+        lin[tid] = act ? i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2] : NONE;
+        lout[tid] = act ? i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2] : NONE;
+        li0[tid] = i0;
+    }
+    __syncthreads();
+    const cx<float> *in = reinterpret_cast<const cx<float> *>(a.in);
     cx<float> *out = reinterpret_cast<cx<float> *>(a.out);
     const cx<float> *tw = reinterpret_cast<const cx<float> *>(a.tw);
     const bool swi = a.flags & FK_SWAP_IN, swo = a.flags & FK_SWAP_OUT;
+
+    /* load the tile */
+    if (a.is == 1) {
+        for (int e = tid; e < NL * L; e += NT) {
+            const int lt = e / L, j = e % L;
+            const long long base = lin[lt];
+            cx<float> v = base != NONE ? in[base + j] : mk(0.f, 0.f);
+            lds[idx(lt, j)] = swi ? swp(v) : v;
+        }
+    } else {
+        for (int e = tid; e < NL * L; e += NT) {
+            const int lt = e % NL, j = e / NL;
+            const long long base = lin[lt];
+            cx<float> v = base != NONE ? in[base + (long long)j * a.is] : mk(0.f, 0.f);
+            lds[idx(lt, j)] = swi ? swp(v) : v;
+        }
+    }
 
     /* A operand (DFT matrix), hi/lo, for the 4 K-steps */
     h4 Ahi[4], Alo[4];
@@ -104,76 +144,109 @@ __global__ void __launch_bounds__(64 * ((S == 3) ? 8 : 1)) k_mfma(fk_args a)
             Ahi[s][qq] = hi;
             Alo[s][qq] = lo;
         }
+    __syncthreads();
 
-    const int b = wave * 32 + n;        /* this lane's butterfly */
-    const bool bval = b < NB;
 #pragma unroll
     for (int st = 0; st < S; ++st) {
         const int Ns = st == 0 ? 1 : st == 1 ? 16 : 256;
-        const int k = bval ? b % Ns : 0;
-        cx<float> x[8];
-        /* inputs r = 4s + 2h + t */
+        const int toff = st == 1 ? 0 : 16 * 15;
+        cx<float> x[GPW][8];
+        /* inputs r = 4s + 2h + t of this lane's butterfly in each group */
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int gi = 0; gi < GPW; ++gi) {
+            const int g = wave + gi * W;
+            if (g < NG) {
+                const int id = g * 32 + n, lt = id / NB, b = id % NB, k = b % Ns;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int r = 4 * s + 2 * h + t;
-                const int j = b + r * NB;
-                cx<float> v = mk(0.f, 0.f);
-                if (bval) {
-                    if (st == 0) {
-                        if (act) v = in[j * a.is];
-                        if (swi) v = swp(v);
-                    } else {
-                        v = lds[idx(j)];
-                        if (r) v = mul(v, tw[(st == 1 ? 0 : 16 * 15) + k * 15 + (r - 1)]);
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int r = 4 * s + 2 * h + t;
+                        cx<float> v = lds[idx(lt, b + r * NB)];
+                        if (st > 0 && r) v = mul(v, tw[toff + k * 15 + (r - 1)]);
+                        x[gi][2 * s + t] = v;
                     }
-                }
-                x[2 * s + t] = v;
             }
-        /* per-column power-of-two scale (the column lives in lanes n, n+32) */
-        float m = 0.f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) m = fmaxf(m, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        const int e = m > 0.f ? ilogbf(m) : 0;
-        const float sc = ldexpf(1.f, 8 - e), usc = ldexpf(1.f, e - 8);
-        f16v acc;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            h4 Bhi, Blo;
-            _Float16 hh, ll;
-            split(x[2 * s].x * sc, hh, ll); Bhi[0] = hh; Blo[0] = ll;
-            split(x[2 * s].y * sc, hh, ll); Bhi[1] = hh; Blo[1] = ll;
-            split(x[2 * s + 1].x * sc, hh, ll); Bhi[2] = hh; Blo[2] = ll;
-            split(x[2 * s + 1].y * sc, hh, ll); Bhi[3] = hh; Blo[3] = ll;
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Ahi[s], Bhi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Ahi[s], Blo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Alo[s], Bhi, acc, 0, 0, 0);
         }
         /* every lane of the workgroup has read this stage's inputs */
-        if (st > 0) __syncthreads();
-        /* outputs: rows 8j + 4h + u, u = 0..3 -> complex kk = 4j + 2h + u/2 */
-        const int base = (b - k) * 16 + k;
+        __syncthreads();
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj)
+        for (int gi = 0; gi < GPW; ++gi) {
+            const int g = wave + gi * W;
+            if (g < NG) {
+                const int id = g * 32 + n, lt = id / NB, b = id % NB, k = b % Ns;
+                /* per-column power-of-two scale (the column lives in lanes
+                   n and n+32) */
+                float m = 0.f;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int kk = 4 * jj + 2 * h + t;
-                cx<float> y = mk(acc[4 * jj + 2 * t] * usc, acc[4 * jj + 2 * t + 1] * usc);
-                const int pos = base + kk * Ns;
-                if (!bval) continue;
-                if (st < S - 1) {
-                    lds[idx(pos)] = y;
-                } else if (act) {
-                    if (a.flags & FK_TW_OUT) y = mul(y, tw2<float>(a, (long long)pos * i0));
-                    if (a.flags & FK_SCALE) { y.x *= (float)a.scale; y.y *= (float)a.scale; }
-                    out[bout + pos * a.os] = swo ? swp(y) : y;
+                for (int i = 0; i < 8; ++i)
+                    m = fmaxf(m, fmaxf(fabsf(x[gi][i].x), fabsf(x[gi][i].y)));
+                m = fmaxf(m, __shfl_xor(m, 32));
+                const int e = m > 0.f ? ilogbf(m) : 0;
+                const float sc = ldexpf(1.f, 8 - e), usc = ldexpf(1.f, e - 8);
+                f16v acc;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    h4 Bhi, Blo;
+                    _Float16 hh, ll;
+                    split(x[gi][2 * s].x * sc, hh, ll); Bhi[0] = hh; Blo[0] = ll;
+                    split(x[gi][2 * s].y * sc, hh, ll); Bhi[1] = hh; Blo[1] = ll;
+                    split(x[gi][2 * s + 1].x * sc, hh, ll); Bhi[2] = hh; Blo[2] = ll;
+                    split(x[gi][2 * s + 1].y * sc, hh, ll); Bhi[3] = hh; Blo[3] = ll;
+                    acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Ahi[s], Bhi, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Ahi[s], Blo, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x8f16(Alo[s], Bhi, acc, 0, 0, 0);
                 }
+                /* outputs: rows 8j + 4h + u, u = 0..3 -> complex kk = 4j + 2h + u/2 */
+                const int base = (b - k) * 16 + k;
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int kk = 4 * jj + 2 * h + t;
+                        lds[idx(lt, base + kk * Ns)] =
+                            mk(acc[4 * jj + 2 * t] * usc, acc[4 * jj + 2 * t + 1] * usc);
+                    }
             }
-        if (st < S - 1) __syncthreads();
+        }
+        __syncthreads();
+    }
+
+    /* store the tile with the output post-op */
+    const bool twout = a.flags & FK_TW_OUT, scl = a.flags & FK_SCALE;
+    if (a.os == 1) {
+        for (int e = tid; e < NL * L; e += NT) {
+            const int lt = e / L, k = e % L;
+            const long long base = lout[lt];
+            if (base == NONE) continue;
+            cx<float> y = lds[idx(lt, k)];
+            if (twout) y = mul(y, tw2<float>(a, (long long)k * li0[lt]));
+            if (scl) { y.x *= (float)a.scale; y.y *= (float)a.scale; }
+            out[base + k] = swo ? swp(y) : y;
+        }
+    } else {
+        /* lines fastest: this thread keeps line lt and steps k by NT/NL, so
+           the four-step twiddle W_N^(k*i0) is looked up in the FP64 table
+           every 16th element and stepped by W_N^((NT/NL)*i0) in FP64 in
+           between (at most 15 FP64 products from a table value) */
+        constexpr int DK = NT / NL;
+        const int lt = tid % NL;
+        const long long base = lout[lt], i0 = li0[lt];
+        cx<double> wcur = mk(1.0, 0.0), wstep = mk(1.0, 0.0);
+        if (twout) wstep = tw2d(a, (long long)DK * i0);
+        int it = 0;
+        for (int k = tid / NL; k < L; k += DK, ++it) {
+            if (base == NONE) break;
+            cx<float> y = lds[idx(lt, k)];
+            if (twout) {
+                wcur = (it % 16 == 0) ? tw2d(a, (long long)k * i0) : mul(wcur, wstep);
+                y = mul(y, mk((float)wcur.x, (float)wcur.y));
+            }
+            if (scl) { y.x *= (float)a.scale; y.y *= (float)a.scale; }
+            out[base + (long long)k * a.os] = swo ? swp(y) : y;
+        }
     }
 }
 
@@ -212,14 +285,17 @@ int fkm_mfma_kernel(int L, fk_kernel *k)
     k->L = L;
     k->nrad = S;
     for (i = 0; i < 12; ++i) k->rad[i] = i < S ? 16 : 0;
-    k->tpl = S == 3 ? 512 : 64;
-    k->nl = 1;
+    /* tiles: 64 lines of 16 (128 threads), 16 lines of 256 (256
+       threads), 2 lines of 4096 (512 threads); block = tpl * nl */
+    k->nl = S == 1 ? 64 : S == 2 ? 16 : 2;
+    k->tpl = (S == 1 ? 128 : S == 2 ? 256 : 512) / k->nl;
     k->mode = FK_ROW;
     k->io = FK_IO_PLAIN;
     k->lds = 0;
     k->twsz = S == 1 ? 0 : S == 2 ? 16 * 15 : 16 * 15 + 256 * 15;
-    k->fn = S == 1 ? (const void *)&k_mfma<1> : S == 2 ? (const void *)&k_mfma<2>
-                                                     : (const void *)&k_mfma<3>;
+    k->fn = S == 1 ? (const void *)&k_mfma<1, 64, 128>
+          : S == 2 ? (const void *)&k_mfma<2, 16, 256>
+                   : (const void *)&k_mfma<3, 2, 512>;
     k->variant = 100;
     return 0;
 }

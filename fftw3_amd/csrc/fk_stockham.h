@@ -77,6 +77,20 @@ struct SyncAll {
     __device__ __forceinline__ static void sync() { __syncthreads(); }
 };
 
+/* Wave-level ordering for stages whose LDS exchange stays inside one wave
+   (row mapping with every line's threads in one wave, lines in disjoint
+   LDS rows): a wave's LDS accesses complete in issue order, so only the
+   compiler must be kept from moving accesses across the exchange point.
+   Waves of the workgroup then run their stages independently. */
+struct SyncWave {
+    __device__ __forceinline__ static void sync()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
 /*
  * One Stockham stage.  LoadF(j) -> cx<R> and StoreF(k, cx<R>) abstract the
  * data source and sink; IN_LDS/OUT_LDS say whether they touch LDS, which
@@ -166,7 +180,8 @@ __device__ __forceinline__ long long remap_tile(unsigned xcd)
  * geometries) the extra threads work on a scratch LDS line and never touch
  * global memory, so every thread still meets every barrier.
  */
-template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT>
+template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT,
+          class SY = SyncAll>
 __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                                               unsigned char *smem_raw)
 {
@@ -238,14 +253,16 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             auto gst = [&](int k, cx<R> v) {
                 if (act) out[bout + k * a.os] = post(k, v);
             };
-            run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+            run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
         } else {
             auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
             /* output offsets may be negative (negative strides): inactive
                lines are marked with the one value no offset takes */
             constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
             if (u == 0 && real) lbase[lt] = act ? bout : NONE;
-            run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+            run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
+            /* the store below reads every line of the tile */
+            if constexpr (!std::is_same<SY, SyncAll>::value) __syncthreads();
             /* lines-fastest store: consecutive threads write consecutive
                lines of the tile (unit output batch stride) */
             for (int e = tid; e < NL * L; e += NT) {
@@ -272,7 +289,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 out[bout + (long long)k * a.os] = v;
             }
         };
-        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+        run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
     } else if constexpr (IO == FK_IO_C2R_FULL) {
         /* c2r of length L from bins k <= L/2: X_{L-k} = conj X_k (Im X_0 and,
            for even L, Im X_{L/2} ignored); the inverse DFT as a forward DFT
@@ -297,7 +314,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             if (fl & FK_SCALE) y *= (R)a.scale;
             outr[(long long)k * a.os] = y;
         };
-        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+        run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
     } else if constexpr (IO == FK_IO_DCT4) {
         /* REDFT11 of n = 2L reals: z_m = (x_2m + i x_{n-1-2m}) exp(-i pi (4m+1) / 4n),
            Z = L-point DFT, c_p = Z_p exp(-i pi p / n), y_2p = 2 Re c_p,
@@ -322,7 +339,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             outr[(2LL * p) * a.os] = ye;
             outr[(N2 - 1 - 2LL * p) * a.os] = sn ? -yo : yo;
         };
-        run_stages<R, RL, 0, L, TPL, false, false>(u, tw, lds_at, gld, gst);
+        run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
     } else if constexpr (IO == FK_IO_R2C || IO == FK_IO_R2C_RR) {
         /* real line of 2L samples viewed as L complex z_j = x_2j + i x_2j+1;
            after the L-point DFT Z, X[k] = E[k] + W_2L^k O[k] with
@@ -376,7 +393,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
         };
         auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
-        run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+        run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* bins k = u + i*TPL in groups of 8 (compile-time count): the
            group's twiddle loads are in flight together */
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
@@ -487,7 +504,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             });
         });
-        __syncthreads();
+        SY::sync();
         auto pre = [&](int j) -> cx<R> {
             const cx<R> xk = lds_at(j), xc = lds_at(L - j);
             const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
@@ -521,7 +538,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 outr[(2 * k + 1) * a.os] = v.x;
             }
         };
-        run_stages<R, RL, 0, L, TPL, true, false>(u, tw, lds_at, pre, gst);
+        run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, pre, gst);
     } else if constexpr (IO == FK_IO_BLUE) {
         /* Bluestein (dft/bluestein.c:82-128 conventions): a_j = x_j c_j,
            c_j = exp(-i pi j^2/n); conv = IDFT_L(DFT_L(a) * B),
@@ -538,7 +555,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             return mul(v, chirp[j]);
         };
         auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
-        run_stages<R, RL, 0, L, TPL, false, true>(u, tw, lds_at, gld, lst);
+        run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         auto mid = [&](int j) -> cx<R> { return swp(mul(lds_at(j), Bt[j])); };
         auto gst = [&](int k, cx<R> v) {
             if (k < n && act) {
@@ -547,7 +564,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 out[bout + k * a.os] = swo ? swp(y) : y;
             }
         };
-        run_stages<R, RL, 0, L, TPL, true, false>(u, tw, lds_at, mid, gst);
+        run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, mid, gst);
     }
 }
 
@@ -556,7 +573,11 @@ __global__ void __launch_bounds__(TPL * NL)
 stockham(fk_args a)
 {
     extern __shared__ unsigned char smem_raw[];
-    stockham_body<R, NL, MODE, IO, PAD, RL, TPL, TPL * NL>(a, remap_tile(a.xcd_remap), smem_raw);
+    /* row mapping with whole lines inside one wave: wave-level exchange */
+    constexpr bool wave_lines = MODE == FK_ROW && TPL <= 64 && 64 % TPL == 0;
+    using SY = typename std::conditional<wave_lines, SyncWave, SyncAll>::type;
+    stockham_body<R, NL, MODE, IO, PAD, RL, TPL, TPL * NL, SY>(a, remap_tile(a.xcd_remap),
+                                                               smem_raw);
 }
 
 template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT = TPL * NL>
