@@ -458,7 +458,7 @@ class Batch:
 
 
 IO_NAMES = ["plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r", "dct4", "r2c-full",
-            "c2r-full"]
+            "c2r-full", "r2c-xr", "c2r-xr"]
 
 
 def catalog():

@@ -1172,6 +1172,175 @@ static int real_pairs(int nb, const fk_dim *b, int out)
     return 1;
 }
 
+/* Rank >= 2 real transforms whose last dimension has a cross-row kernel
+   (FK_IO_R2C_XR / FK_IO_C2R_XR, X = its rows per tile): dimension r-2
+   (n0 = X*m) is split as n = n1 + m*n2 (input rows) / k = X*q + p (output
+   rows).  The row pass transforms the X rows n1 + m*n2 of a tile and runs
+   the first stage of their column DFT,
+       y_p[k] = W_n0^(n1*p) sum_n2 X_(n1+m*n2)[k] W_X^(n2*p),
+   into row X*n1 + p; what is left of dimension r-2 is an m-point DFT over
+   rows X apart, in place: rows X*q + p (q = 0..m-1) -> X*q + p, so its
+   column tiles hold X times more lines than an n0-point column pass could
+   and read X times longer runs.  This is the four-step split of
+   dft/ct.c:34-45 applied across the rank split of rdft/rank-geq2-rdft2.c
+   (rows first, then the other dimensions on n/2+1 columns).  c2r mirrors
+   it: the m-point inverse over rows X apart first, in place on the input
+   (which multi-dimensional c2r may destroy), then the last inverse stage
+   across the tile's rows fused into the c2r row pass.  Returns the kernel
+   and the extra batch dim (dims 0..r-3 and the howmany dims merged into at
+   most one), or NULL when the problem does not fit. */
+static const fk_kernel *xr_kernel(ctx *c, const fk_problem *pb, int io, fk_dim *extra, int *nextra)
+{
+    int r = pb->rank, i, ne = 0;
+    long long nl = pb->dims[r > 0 ? r - 1 : 0].n, n0;
+    const fk_kernel *k;
+    fk_dim e[FK_MAXRANK * 2];
+    if (r < 2 || nl % 2 || pb->in == pb->out || c->r16) return 0;
+    k = find_kernel(c->P->prec, nl / 2, FK_ROW, io);
+    if (!k) return 0;
+    n0 = pb->dims[r - 2].n;
+    if (n0 % k->nl || n0 / k->nl < 2) return 0;
+    for (i = 0; i < r - 2; ++i) e[ne++] = pb->dims[i];
+    for (i = 0; i < pb->vrank; ++i) e[ne++] = pb->vdims[i];
+    ne = squeeze(ne, e);
+    ne = merge_dims(ne, e);
+    if (ne > 1) return 0;
+    if (ne) extra[0] = e[0];
+    *nextra = ne;
+    return k;
+}
+
+static int plan_r2c_xr(ctx *c, const fk_problem *pb, const fk_kernel *kx, const fk_dim *extra,
+                       int nextra)
+{
+    fk_plan *P = c->P;
+    int prec = P->prec, r = pb->rank, i, nb, nv = 0;
+    size_t cs = csz(prec), rs = rsz(prec), before = (size_t)P->npass;
+    const fk_dim *d0 = &pb->dims[r - 2], *dl = &pb->dims[r - 1];
+    long long nl = dl->n, nh = nl / 2 + 1, L = nl / 2, X = kx->nl, n0 = d0->n, m = n0 / X, tb = 0;
+    fk_dim b[3], lead[FK_MAXRANK], vd[FK_MAXRANK * 2];
+    const void *wk = line_table(prec, nl, L + 1, c->st);
+    const double *t2 = tw2_table(n0, &tb, c->st);
+    unsigned fl = 0;
+    int unit;
+    if (!wk || !t2) {
+        c->err = 1;
+        set_err("twiddle table allocation failed");
+        return -1;
+    }
+    /* tile = the X rows n1 + m*n2 of one n1: batch dim 0 must be n2 */
+    b[0].n = X;
+    b[0].is = m * d0->is;
+    b[0].os = d0->os;
+    b[1].n = m;
+    b[1].is = d0->is;
+    b[1].os = X * d0->os;
+    nb = 2;
+    if (nextra) b[nb++] = extra[0];
+    unit = dl->is == 1 && ((uintptr_t)pb->in % (2 * rs)) == 0;
+    for (i = 0; i < nb; ++i) unit &= (b[i].is % 2) == 0;
+    if (unit) fl |= FK_REAL_UNIT;
+    emit_stockham(c, kx, dl->is, dl->os, nb, b, c->bin, 0, c->bout, 0, fl, rs, cs);
+    for (; before < (size_t)P->npass; ++before) {
+        fk_pass *p = P->pass + before;
+        p->a.aux = wk;
+        p->a.tlo = t2;
+        p->a.thi = t2 + 2 * tb;
+        p->a.tN = n0;
+        p->a.tb = tb;
+        p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+    }
+    if (unit) need_align(c, c->bin, (unsigned)(2 * rs));
+    desc_add(P, "(r2c-rows+radix-%lld L=%lld)", X, L);
+    if (c->err) return -1;
+    /* dims 0..r-3 and the m-point rest of dim r-2, in place on the output */
+    for (i = 0; i < r - 2; ++i) {
+        lead[i].n = pb->dims[i].n;
+        lead[i].is = lead[i].os = pb->dims[i].os;
+    }
+    lead[r - 2].n = m;
+    lead[r - 2].is = lead[r - 2].os = X * d0->os;
+    vd[nv].n = nh;
+    vd[nv].is = vd[nv].os = dl->os;
+    ++nv;
+    vd[nv].n = X;
+    vd[nv].is = vd[nv].os = d0->os;
+    ++nv;
+    for (i = 0; i < pb->vrank; ++i) {
+        vd[nv].n = pb->vdims[i].n;
+        vd[nv].is = vd[nv].os = pb->vdims[i].os;
+        ++nv;
+    }
+    return plan_c2c_dims(c, r - 1, lead, nv, vd, c->bout, c->bout, 0);
+}
+
+static int plan_c2r_xr(ctx *c, const fk_problem *pb, const fk_kernel *kx, const fk_dim *extra,
+                       int nextra)
+{
+    fk_plan *P = c->P;
+    int prec = P->prec, r = pb->rank, i, nb, nv = 0;
+    size_t cs = csz(prec), rs = rsz(prec), before;
+    const fk_dim *d0 = &pb->dims[r - 2], *dl = &pb->dims[r - 1];
+    long long nl = dl->n, nh = nl / 2 + 1, L = nl / 2, X = kx->nl, n0 = d0->n, m = n0 / X, tb = 0;
+    fk_dim b[3], lead[FK_MAXRANK], vd[FK_MAXRANK * 2];
+    const void *wk = line_table(prec, nl, L + 1, c->st);
+    const double *t2 = tw2_table(n0, &tb, c->st);
+    unsigned fl = 0;
+    int unit;
+    if (!wk || !t2) {
+        c->err = 1;
+        set_err("twiddle table allocation failed");
+        return -1;
+    }
+    /* dims 0..r-3 and the m-point inverse over rows X apart, in place on
+       the input (api/plan-many-dft-c2r.c:41-42: it may be destroyed) */
+    for (i = 0; i < r - 2; ++i) {
+        lead[i].n = pb->dims[i].n;
+        lead[i].is = lead[i].os = pb->dims[i].is;
+    }
+    lead[r - 2].n = m;
+    lead[r - 2].is = lead[r - 2].os = X * d0->is;
+    vd[nv].n = nh;
+    vd[nv].is = vd[nv].os = dl->is;
+    ++nv;
+    vd[nv].n = X;
+    vd[nv].is = vd[nv].os = d0->is;
+    ++nv;
+    for (i = 0; i < pb->vrank; ++i) {
+        vd[nv].n = pb->vdims[i].n;
+        vd[nv].is = vd[nv].os = pb->vdims[i].is;
+        ++nv;
+    }
+    if (plan_c2c_dims(c, r - 1, lead, nv, vd, c->bin, c->bin, 1)) return -1;
+    /* tile = the X input rows X*n1 + p of one n1 (batch dim 0 = p); line
+       n2 of the tile is output row n1 + m*n2 */
+    b[0].n = X;
+    b[0].is = d0->is;
+    b[0].os = m * d0->os;
+    b[1].n = m;
+    b[1].is = X * d0->is;
+    b[1].os = d0->os;
+    nb = 2;
+    if (nextra) b[nb++] = extra[0];
+    unit = dl->os == 1 && ((uintptr_t)pb->out % (2 * rs)) == 0;
+    for (i = 0; i < nb; ++i) unit &= (b[i].os % 2) == 0;
+    if (unit) fl |= FK_REAL_UNIT;
+    before = (size_t)P->npass;
+    emit_stockham(c, kx, dl->is, dl->os, nb, b, c->bin, 0, c->bout, 0, fl, cs, rs);
+    for (; before < (size_t)P->npass; ++before) {
+        fk_pass *p = P->pass + before;
+        p->a.aux = wk;
+        p->a.tlo = t2;
+        p->a.thi = t2 + 2 * tb;
+        p->a.tN = n0;
+        p->a.tb = tb;
+        p->bytes = (double)batch_count(nb, b) * ((double)nl * rs + (double)nh * cs);
+    }
+    if (unit) need_align(c, c->bout, (unsigned)(2 * rs));
+    desc_add(P, "(radix-%lld+c2r-rows L=%lld)", X, L);
+    return c->err ? -1 : 0;
+}
+
 static int plan_r2c(ctx *c, const fk_problem *pb)
 {
     fk_plan *P = c->P;
@@ -1186,6 +1355,12 @@ static int plan_r2c(ctx *c, const fk_problem *pb)
     for (i = 0; i < pb->vrank; ++i) b[nb++] = pb->vdims[i];
     nb = squeeze(nb, b);
     sort_by_is(nb, b);
+    {
+        fk_dim xe[1];
+        int nxe = 0;
+        const fk_kernel *kx = xr_kernel(c, pb, FK_IO_R2C_XR, xe, &nxe);
+        if (kx && decide(c, 2, DEC_VARIANT) == 0) return plan_r2c_xr(c, pb, kx, xe, nxe);
+    }
     if (nl % 2 == 0) k = pick_variant(c, prec, L, FK_ROW, FK_IO_R2C);
     if (k) {
         size_t before = (size_t)P->npass;
@@ -1316,6 +1491,12 @@ static int plan_c2r(ctx *c, const fk_problem *pb)
     fk_dim b[FK_MAXRANK * 2], lead[FK_MAXRANK];
     int nb = 0;
     const fk_kernel *k = 0;
+    {
+        fk_dim xe[1];
+        int nxe = 0;
+        const fk_kernel *kx = xr_kernel(c, pb, FK_IO_C2R_XR, xe, &nxe);
+        if (kx && decide(c, 2, DEC_VARIANT) == 0) return plan_c2r_xr(c, pb, kx, xe, nxe);
+    }
     /* leading dims first: backward c2c in place on the input (destroys it,
        api/plan-many-dft-c2r.c:41-42) */
     if (r > 1) {
@@ -2340,6 +2521,7 @@ int fk_plan_use_mfma(fk_plan *p)
         int all16 = 1;
         if (ps->type != FK_PASS_STOCKHAM || k->io == FK_IO_R2C || k->io == FK_IO_C2R ||
             k->io == FK_IO_BLUE || k->io == FK_IO_R2C_RR || k->io == FK_IO_C2R_RR ||
+            k->io == FK_IO_R2C_XR || k->io == FK_IO_C2R_XR ||
             k->io == FK_IO_DCT4 || k->io == FK_IO_R2C_FULL || k->io == FK_IO_C2R_FULL)
             continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
@@ -2509,8 +2691,8 @@ static const char *buf_name(int b)
 static const char *io_name(int io)
 {
     static const char *n[] = {"plain", "tout", "r2c", "c2r", "bluestein", "r2c-r2r", "c2r-r2r",
-                              "dct4", "r2c-full", "c2r-full"};
-    return io >= 0 && io < 10 ? n[io] : "?";
+                              "dct4", "r2c-full", "c2r-full", "r2c-xr", "c2r-xr"};
+    return io >= 0 && io < 12 ? n[io] : "?";
 }
 
 int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)

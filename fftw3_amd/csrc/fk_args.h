@@ -25,7 +25,11 @@ enum {
     FK_IO_C2R_RR = 6, /* FK_IO_C2R with a fused real-to-real kind (rmode) */
     FK_IO_DCT4 = 7,   /* REDFT11 / RODFT11 of n = 2L reals through an L-point DFT */
     FK_IO_R2C_FULL = 8, /* real line of n = L samples, L-point DFT, bins k <= L/2 stored */
-    FK_IO_C2R_FULL = 9  /* bins k <= L/2, Hermitian L-point inverse, real part stored */
+    FK_IO_C2R_FULL = 9, /* bins k <= L/2, Hermitian L-point inverse, real part stored */
+    FK_IO_R2C_XR = 10,  /* FK_IO_R2C on the NL rows of a tile, then the first (radix-NL,
+                           decimation in frequency) stage of the column DFT across them */
+    FK_IO_C2R_XR = 11   /* the last (radix-NL, decimation in time) stage of the inverse
+                           column DFT across the NL rows of a tile, then FK_IO_C2R */
 };
 
 /* flags */

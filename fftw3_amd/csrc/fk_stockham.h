@@ -340,7 +340,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             outr[(N2 - 1 - 2LL * p) * a.os] = sn ? -yo : yo;
         };
         run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
-    } else if constexpr (IO == FK_IO_R2C || IO == FK_IO_R2C_RR) {
+    } else if constexpr (IO == FK_IO_R2C || IO == FK_IO_R2C_RR || IO == FK_IO_R2C_XR) {
         /* real line of 2L samples viewed as L complex z_j = x_2j + i x_2j+1;
            after the L-point DFT Z, X[k] = E[k] + W_2L^k O[k] with
            E = (Z[k] + conj Z[L-k])/2, O = (Z[k] - conj Z[L-k])/(2i),
@@ -398,6 +398,14 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
            group's twiddle loads are in flight together */
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
         constexpr int GK = 8;
+        constexpr bool XR = IO == FK_IO_R2C_XR;
+        /* cross-row form: the bins stay in registers until every bin of the
+           line has been formed, then replace Z in the line's LDS row */
+        cx<R> xs[XR ? NK : 1];
+        constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
+        if constexpr (XR) {
+            if (u == 0 && real) lbase[lt] = act ? bout : NONE;
+        }
         sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
@@ -418,7 +426,9 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
                     cx<R> X = mk(er + orr * w[i].x - oi * w[i].y, ei + orr * w[i].y + oi * w[i].x);
                     if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
-                    if (act && rm == FK_RM_NONE) {
+                    if constexpr (XR) {
+                        xs[g * GK + i] = X;
+                    } else if (act && rm == FK_RM_NONE) {
                         out[bout + (long long)k * a.os] = X;
                     } else if (act && rm == FK_RM_E00) {
                         outr[(long long)k * a.os] = X.x;             /* DCT-I: Re X_k */
@@ -451,7 +461,41 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             });
         });
-    } else if constexpr (IO == FK_IO_C2R || IO == FK_IO_C2R_RR) {
+        if constexpr (XR) {
+            /* every thread of the line has read Z: the bins replace it */
+            SY::sync();
+            sfor<0, NK>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + i * TPL;
+                if (k <= L) lds_at(k) = xs[i];
+            });
+            __syncthreads();
+            /* first stage of the column DFT over the tile's NL rows
+               n = n1 + m*n2 (n2 = the line's batch index i0, n1 = i1; the
+               planner sets nb[0] = NL, so a tile is one n1): for every bin k
+               y_p = W_n0^(n1*p) sum_n2 X_n2[k] W_NL^(n2*p), n0 = a.tN,
+               stored to the tile's output line p */
+            const long long n1 = tile % a.nb[1];
+            cx<R> wp[NL];
+            sfor<0, NL>([&](auto pc) {
+                constexpr int pp = decltype(pc)::value;
+                wp[pp] = pp == 0 ? mk((R)1, (R)0) : tw2<R>(a, n1 * pp);
+            });
+            for (int k = tid; k <= L; k += NT) {
+                cx<R> v[NL];
+                sfor<0, NL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    v[q] = lds[G::idx(q, k)];
+                });
+                Dft<R, NL, 1>::run(v);
+                sfor<0, NL>([&](auto pc) {
+                    constexpr int pp = decltype(pc)::value;
+                    const long long ob = lbase[pp];
+                    if (ob != NONE) out[ob + (long long)k * a.os] = pp == 0 ? v[0] : mul(v[pp], wp[pp]);
+                });
+            }
+        }
+    } else if constexpr (IO == FK_IO_C2R || IO == FK_IO_C2R_RR || IO == FK_IO_C2R_XR) {
         /* half-length inverse: Z[k] = E[k] + i O[k],
            E[k] = X[k] + conj X[L-k], O[k] = (X[k] - conj X[L-k]) W_2L^-k,
            z = DFT^-1_L(Z) (unnormalized), y_2m = Re z_m, y_2m+1 = Im z_m.
@@ -479,10 +523,46 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             const cx<R> w = rt[2 * k];
             return mk(w.x * xk - w.y * xn, -(w.x * xn + w.y * xk));
         };
-        /* loads in groups of 8 (compile-time count) so that each group is
-           in flight at once, then its LDS stores */
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
         constexpr int GK = 8;
+        if constexpr (IO == FK_IO_C2R_XR) {
+            /* last stage of the inverse column DFT over the tile's NL input
+               rows (row p = the line's batch index i0, n1 = i1; the planner
+               sets nb[0] = NL, so a tile is one n1): for every bin k
+               u_n2 = sum_p W_NL^(-n2*p) W_n0^(-n1*p) Y_p[k], n0 = a.tN,
+               which is line n2's half spectrum (its own output row) */
+            constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
+            if (u == 0 && real) lbase[lt] = act ? bin : NONE;
+            const long long n1 = tile % a.nb[1];
+            cx<R> wp[NL];
+            sfor<0, NL>([&](auto pc) {
+                constexpr int pp = decltype(pc)::value;
+                const cx<R> t = pp == 0 ? mk((R)1, (R)0) : tw2<R>(a, n1 * pp);
+                wp[pp] = mk(t.x, -t.y);
+            });
+            __syncthreads();
+            const cx<R> *inb = reinterpret_cast<const cx<R> *>(a.in);
+            for (int k = tid; k <= L; k += NT) {
+                cx<R> v[NL];
+                sfor<0, NL>([&](auto pc) {
+                    constexpr int pp = decltype(pc)::value;
+                    const long long ib = lbase[pp];
+                    const cx<R> y = ib != NONE ? inb[ib + (long long)k * a.is] : mk((R)0, (R)0);
+                    /* inverse DFT through the forward one on swapped data */
+                    v[pp] = swp(pp == 0 ? y : mul(y, wp[pp]));
+                });
+                Dft<R, NL, 1>::run(v);
+                sfor<0, NL>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    cx<R> z = swp(v[q]);
+                    if (k == 0 || k == L) z.y = 0;
+                    lds[G::idx(q, k)] = z;
+                });
+            }
+            __syncthreads();
+        } else {
+        /* loads in groups of 8 (compile-time count) so that each group is
+           in flight at once, then its LDS stores */
         sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
@@ -505,6 +585,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             });
         });
         SY::sync();
+        }
         auto pre = [&](int j) -> cx<R> {
             const cx<R> xk = lds_at(j), xc = lds_at(L - j);
             const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);

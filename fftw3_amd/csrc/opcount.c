@@ -140,11 +140,22 @@ static void stockham_ops(const fk_pass *ps, ops *o)
     switch (k->io) {
     case FK_IO_R2C:
     case FK_IO_R2C_RR:
+    case FK_IO_R2C_XR:
         /* L+1 bins: E and O (4 add, 4 mul by 1/2), X = E + W O (4 fma) */
         outs = L + 1;
         t.add += 4 * outs;
         t.mul += 4 * outs;
         t.fma += 4 * outs;
+        if (k->io == FK_IO_R2C_XR) {
+            /* per bin and row: 1/nl of a DFT_nl across the tile's rows and
+               (nl-1)/nl of a twiddle product */
+            double da, dm, df;
+            fk_dft_ops(k->nl, &da, &dm, &df);
+            t.add += outs * da / k->nl;
+            t.mul += outs * dm / k->nl;
+            t.fma += outs * df / k->nl;
+            ops_acc(&t, outs * (k->nl - 1) / k->nl, &CMUL);
+        }
         if (k->io == FK_IO_R2C_RR) {
             const unsigned rm = ps->a.rmode;
             if (rm == FK_RM_DHT) t.add += 2 * outs;
@@ -156,11 +167,21 @@ static void stockham_ops(const fk_pass *ps, ops *o)
         break;
     case FK_IO_C2R:
     case FK_IO_C2R_RR:
+    case FK_IO_C2R_XR:
         /* pre-pass per complex input: E, D (4 add), conj-twiddle product
            (2 mul + 2 fma), Z = E + i O (2 add) */
         t.add += 6 * L;
         t.mul += 2 * L;
         t.fma += 2 * L;
+        if (k->io == FK_IO_C2R_XR) {
+            /* per bin and row: the cross-row inverse stage, as in R2C_XR */
+            double da, dm, df;
+            fk_dft_ops(k->nl, &da, &dm, &df);
+            t.add += (L + 1) * da / k->nl;
+            t.mul += (L + 1) * dm / k->nl;
+            t.fma += (L + 1) * df / k->nl;
+            ops_acc(&t, (L + 1) * (k->nl - 1) / k->nl, &CMUL);
+        }
         if (k->io == FK_IO_C2R_RR &&
             (ps->a.rmode == FK_RM_DCT3 || ps->a.rmode == FK_RM_DST3)) {
             /* exp(+i pi k / 2n)(x_k - i x_{n-k}) per bin */

@@ -23,6 +23,7 @@ LDS_TARGET = 64 * 1024        # keep >= 2 workgroups per CU when possible
 CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
+XR_ROWS = 8                   # rows per tile of the cross-row real kernels
 
 
 def factorizations(n, maxr=16):
@@ -195,6 +196,18 @@ def catalog():
                         gc = geometry(prec, L, rads, "COL", io)
                         if gc:
                             entries.append((prec, L, rads, "COL", io + "_RR", gc))
+            # half-length r2c / c2r fused with one radix-8 stage of the next
+            # dimension's DFT across the 8 rows of a tile (2-D and higher
+            # real transforms): the remaining column pass is 8x shorter, so
+            # its tiles hold 8x more lines and read longer runs
+            if (prec, L) in (("float", 512), ("float", 1024), ("double", 512)):
+                g = geometry(prec, L, rads, "ROW", "R2C")
+                if g:
+                    lp = L + L // g["pad"] + 2
+                    gx = dict(g, nl=XR_ROWS, lds=lp * XR_ROWS * CSZ[prec] + 8 * XR_ROWS)
+                    if gx["lds"] <= LDS_MAX and gx["tpl"] * XR_ROWS <= 1024:
+                        entries.append((prec, L, rads, "ROW", "R2C_XR", gx))
+                        entries.append((prec, L, rads, "ROW", "C2R_XR", gx))
             # full-length real kernels for odd lengths (no half-length form)
             if L % 2 == 1 and L <= 8192:
                 for io in ("R2C_FULL", "C2R_FULL"):

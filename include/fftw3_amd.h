@@ -77,7 +77,9 @@ double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *pe
    length L, mapping (0 row, 1 column), IO kind (0 plain, 1 transposed
    store, 2 r2c, 3 c2r, 4 Bluestein, 5 r2c with a real-to-real kind, 6 c2r
    with a real-to-real kind, 7 DCT-IV, 8 full-length r2c, 9 full-length
-   c2r), threads per line, lines per workgroup, variant (0 = the default
+   c2r, 10 r2c with the first radix-(lines per workgroup) stage of the next
+   dimension across the workgroup's rows, 11 c2r with the last inverse such
+   stage), threads per line, lines per workgroup, variant (0 = the default
    geometry), number of radices, the radices (12 slots), dynamic LDS bytes.
    Returns 0, or -1 for an index out of range. */
 int fftw_amd_catalog_size(void);

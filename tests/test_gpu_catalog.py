@@ -130,6 +130,35 @@ def run_c2r(e, n, H, strided):
     return y, ref
 
 
+def run_2d_real(e, n0, n1, H, forward):
+    """2-D r2c (forward) or c2r of n0 x n1 reals, howmany H, out of place."""
+    prec = e["prec"]
+    nh = n1 // 2 + 1
+    if forward:
+        x = oracle.uniform(n0 * n1 * H, seed_of(e)).astype(rdt(prec))
+        bi = fa.DeviceBuffer.from_array(x)
+        bo = fa.DeviceBuffer(n0 * nh * H * np.dtype(cdt(prec)).itemsize)
+        p = fa.plan_many_dft_r2c(prec, [n0, n1], H, bi.ptr, None, 1, n0 * n1, bo.ptr, None, 1, n0 * nh)
+        check_uses(p, e)
+        p.execute()
+        y = bo.download(cdt(prec), n0 * nh * H)
+        ref = oracle.rfftn(x.astype(np.float64).reshape(H, n0, n1), axes_batch=1).reshape(-1)
+    else:
+        # Hermitian input: the r2c of real data (the c2r reads only that half)
+        xr = oracle.uniform(n0 * n1 * H, seed_of(e)).reshape(H, n0, n1)
+        X = oracle.rfftn(xr, axes_batch=1)
+        x = X.astype(cdt(prec)).reshape(-1)
+        bi = fa.DeviceBuffer.from_array(x)
+        bo = fa.DeviceBuffer(n0 * n1 * H * np.dtype(rdt(prec)).itemsize)
+        p = fa.plan_many_dft_c2r(prec, [n0, n1], H, bi.ptr, None, 1, n0 * nh, bo.ptr, None, 1, n0 * n1)
+        check_uses(p, e)
+        p.execute()
+        y = bo.download(rdt(prec), n0 * n1 * H)
+        ref = oracle.irfftn(x.astype(np.complex128).reshape(H, n0, nh), n1, axes_batch=1).reshape(-1)
+    p.destroy()
+    return y, ref
+
+
 def run_r2r(e, kind, n, H, col):
     prec = e["prec"]
     if col:
@@ -174,6 +203,11 @@ def cases(e):
         out.append(("r2c full", lambda: run_r2c(e, L, H, strided)))
     elif io == "c2r-full":
         out.append(("c2r full", lambda: run_c2r(e, L, H, strided)))
+    elif io in ("r2c-xr", "c2r-xr"):
+        # 2-D problems whose first dim is nl * m (nl = the kernel's rows per
+        # tile), howmany 2: the row pass runs the first radix-nl stage of
+        # the column DFT
+        out.append(("%s 2-D" % io, lambda: run_2d_real(e, e["nl"] * 4, 2 * L, 2, io == "r2c-xr")))
     else:
         col = mode == "col"
         if io == "r2c-r2r":
@@ -233,5 +267,5 @@ def test_catalog_is_complete():
     known = {("row", "plain"), ("col", "plain"), ("row", "tout"), ("row", "bluestein"),
              ("row", "r2c"), ("row", "c2r"), ("row", "r2c-full"), ("row", "c2r-full"),
              ("row", "r2c-r2r"), ("col", "r2c-r2r"), ("row", "c2r-r2r"), ("col", "c2r-r2r"),
-             ("row", "dct4"), ("col", "dct4")}
+             ("row", "dct4"), ("col", "dct4"), ("row", "r2c-xr"), ("row", "c2r-xr")}
     assert CAT and kinds <= known, kinds - known
