@@ -868,6 +868,14 @@ double fftw_amd_plan_time(void *p, void *in, void *out, int iters, double *per_p
     return p ? fk_plan_time((fk_plan *)p, in, out, iters, per_pass_ms) : -1;
 }
 
+/* The new-array execute (execute_dft / _r2c / _c2r / _r2r) with the
+   library's static device scratch in place of the plan's own: allocates
+   nothing.  Returns 0 or a HIP error code (no abort). */
+int fftw_amd_execute_noalloc(const void *p, void *in, void *out)
+{
+    return fk_execute_noalloc((fk_plan *)p, in, out);
+}
+
 int fftw_amd_catalog_size(void) { return fk_catalog_size(); }
 int fftw_amd_catalog_entry(int i, int *info)
 {

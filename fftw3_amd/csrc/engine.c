@@ -72,6 +72,13 @@ int fk_init(void)
             }
         }
     }
+    /* device address of the allocation-free execute's static scratch on the
+       current device (other devices take theirs on first use) */
+    i = fk_static_scratch_setup();
+    if (i) {
+        set_err("static scratch set-up failed (%d)", i);
+        (void)hipGetLastError();
+    }
     g_inited = 1;
     return 0;
 }
@@ -2447,6 +2454,58 @@ static int launch_pass(const fk_pass *ps, void *const *buf, hipStream_t st, int 
 }
 
 int fk_execute(fk_plan *p, void *in, void *out) { return fk_execute4(p, in, 0, out, 0); }
+
+/* The passes of fk_execute4 with every scratch slot carved (256-byte
+   aligned) out of the static device scratch, whose address was taken once
+   per device at set-up.  No allocation on any path: a plan whose slots do
+   not fit is rejected, and split-array plans (two more arrays) are not
+   served.  Synchronizes unless async mode is on. */
+int fk_execute_noalloc(fk_plan *p, void *in, void *out)
+{
+    void *buf[FK_NBUF];
+    void *base = 0;
+    size_t cap = 0, off = 0;
+    int i, rc;
+    if (!p) {
+        set_err("execute_noalloc: NULL plan");
+        return (int)hipErrorInvalidValue;
+    }
+    if (p->split == FK_SPLIT_GENERAL) {
+        set_err("execute_noalloc: split-array plans are not supported");
+        return (int)hipErrorInvalidValue;
+    }
+    if (((uintptr_t)in | (uintptr_t)out) % p->align_need ||
+        (p->align_in && ((uintptr_t)in % p->align_in)) ||
+        (p->align_out && ((uintptr_t)out % p->align_out))) {
+        set_err("execute_noalloc: arrays not aligned as at planning (%u / %u / %u bytes)",
+                p->align_need, p->align_in, p->align_out);
+        return (int)hipErrorInvalidValue;
+    }
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) off += (p->scratch_bytes[i] + 255) & ~(size_t)255;
+    if (off > FK_STATIC_SCRATCH_BYTES) {
+        set_err("execute_noalloc: plan needs %zu bytes of scratch, the static scratch holds %zu",
+                off, (size_t)FK_STATIC_SCRATCH_BYTES);
+        return (int)hipErrorInvalidValue;
+    }
+    rc = fk_static_scratch(&base, &cap);
+    if (rc || !base) {
+        set_err("execute_noalloc: static scratch unavailable (%d)", rc);
+        return rc ? rc : (int)hipErrorInvalidSymbol;
+    }
+    buf[FK_BUF_IN] = in;
+    buf[FK_BUF_OUT] = out;
+    buf[FK_BUF_IN2] = 0;
+    buf[FK_BUF_OUT2] = 0;
+    off = 0;
+    for (i = 0; i < FK_MAX_SCRATCH; ++i) {
+        buf[FK_BUF_S0 + i] = p->scratch_bytes[i] ? (char *)base + off : 0;
+        off += (p->scratch_bytes[i] + 255) & ~(size_t)255;
+    }
+    for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, p->stream, p->prec);
+    if (!rc && !g_async) rc = (int)hipStreamSynchronize(p->stream);
+    if (rc) set_err("execute_noalloc failed: %s", hipGetErrorString((hipError_t)rc));
+    return rc;
+}
 
 int fk_execute4(fk_plan *p, void *in, void *in2, void *out, void *out2)
 {

@@ -127,6 +127,13 @@ int fk_execute(fk_plan *p, void *in, void *out);         /* 0 or hip error */
    otherwise) */
 int fk_execute4(fk_plan *p, void *in, void *in2, void *out, void *out2);
 void fk_execute_or_die(fk_plan *p, void *in, void *out);
+/* Execute with the static device scratch of fk_noalloc.hip in place of the
+   plan's own: allocates nothing on any path.  Calls that overlap on one
+   device share that scratch and race. */
+#define FK_STATIC_SCRATCH_BYTES ((size_t)128 << 20)
+int fk_execute_noalloc(fk_plan *p, void *in, void *out);
+int fk_static_scratch_setup(void);
+int fk_static_scratch(void **ptr, size_t *bytes);
 void fk_cleanup(void);
 void fk_forget_wisdom(void);
 void fk_set_timelimit(double t);

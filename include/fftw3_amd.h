@@ -72,6 +72,24 @@ double fftw_amd_plan_pass_bytes(const void *plan, int i);
    whole execute.  Negative on error. */
 double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *per_pass_ms);
 
+/* Allocation-free execute.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory during the call (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Runs `plan` on new arrays exactly as fftw_execute_dft / _r2c / _c2r /
+   _r2r would, but every scratch buffer of the plan is carved out of one
+   static device array of FFTW_AMD_STATIC_SCRATCH_BYTES (128 MiB, held by
+   each device the library runs on until the process ends) instead of the
+   plan's own scratch, and no allocation happens on any path.  Plans whose
+   scratch exceeds that size, and split-array plans, are rejected with a
+   nonzero HIP error code (fftw_amd_last_error says why).  All calls on one
+   device share the array, which keeps the last call's contents: calls
+   that overlap on one device (other host threads, other streams, or async
+   mode with executes of different plans in flight on different streams)
+   race.  Synchronizes unless fftw_amd_set_async(1).  Returns 0 on
+   success. */
+#define FFTW_AMD_STATIC_SCRATCH_BYTES ((size_t)128 << 20)
+int fftw_amd_execute_noalloc(const void *plan, void *in, void *out);
+
 /* Kernel catalog (the compiled Stockham kernels the planner chooses from).
    fftw_amd_catalog_entry fills info[21]: precision (0 double, 1 float),
    length L, mapping (0 row, 1 column), IO kind (0 plain, 1 transposed
