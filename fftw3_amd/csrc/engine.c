@@ -2308,22 +2308,53 @@ static double now_s(void)
 #define TUNE_MIN_MS 1.0
 #define TUNE_LIMIT_S 2.0
 static double fk_plan_time_run(fk_plan *p, int iter);
-static double time_plan(fk_plan *P)
+static int cmp_dbl(const void *a, const void *b)
 {
-    double t0 = now_s(), tmin = 1e30;
-    int iter, r;
+    double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : x > y;
+}
+
+/* *spread (if given) = (median - minimum) / minimum of the last round's
+   repeats: the run-to-run noise of this candidate */
+static double time_plan(fk_plan *P, double *spread)
+{
+    double t0 = now_s(), tmin = 1e30, ts[TUNE_REPEAT];
+    int iter, r, nr = 0;
+    if (spread) *spread = 0;
     if (fk_plan_time_run(P, 1) < 0) return 1e30; /* warm-up */
     for (iter = 1; iter <= (1 << 16); iter *= 2) {
         tmin = 1e30;
-        for (r = 0; r < TUNE_REPEAT; ++r) {
+        for (r = nr = 0; r < TUNE_REPEAT; ++r) {
             double t = fk_plan_time_run(P, iter);
             if (t < 0) return 1e30;
+            ts[nr++] = t;
             if (t < tmin) tmin = t;
             if (now_s() - t0 > TUNE_LIMIT_S) break;
         }
         if (tmin >= TUNE_MIN_MS || now_s() - t0 > TUNE_LIMIT_S) break;
     }
+    if (spread && nr > 1) {
+        qsort(ts, (size_t)nr, sizeof(double), cmp_dbl);
+        *spread = (ts[nr / 2] - tmin) / tmin;
+    }
     return tmin / iter;
+}
+
+/* A candidate replaces the current plan only when it is faster by more
+   than the larger of 3% and twice the two candidates' combined noise, and a
+   second timing of both confirms it: decisions inside the noise keep the
+   heuristic choice, so repeated FFTW_MEASURE runs reach the same plan. */
+static int clearly_faster(fk_plan *P, double *best, double sp_best, fk_plan *Q, double t,
+                          double sp_t)
+{
+    double thr = 2.0 * (sp_best + sp_t), t2, b2;
+    if (thr < 0.03) thr = 0.03;
+    if (!(t < *best * (1.0 - thr))) return 0;
+    b2 = time_plan(P, 0);
+    t2 = time_plan(Q, 0);
+    if (!(t2 < b2 * (1.0 - thr))) return 0;
+    *best = t < t2 ? t : t2;
+    return 1;
 }
 
 /*
@@ -2340,7 +2371,7 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
     unsigned long long h = problem_hash(pb);
     fk_plan *P, *Q;
     ctx cb, cq;
-    double best, t0;
+    double best, t0, sp_best = 0, sp_t = 0;
     if (fk_wisdom_lookup(h, &wlevel, wc, &wn) &&
         (wlevel >= level || (pb->flags & FK_WISDOM_ONLY))) {
         P = build_plan(pb, st, wc, wn, 0);
@@ -2354,7 +2385,7 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
     P = build_plan(pb, st, 0, 0, &cb);
     if (!P || level == 0 || !P->in || !P->out || P->npass == 0) return P;
     t0 = now_s();
-    best = time_plan(P);
+    best = time_plan(P, &sp_best);
     ncur = P->nchoice;
     memcpy(cur, P->choice, sizeof(int) * ncur);
     for (i = 0; i < ncur && i < FK_MAX_DECIDE; ++i) {
@@ -2368,11 +2399,11 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
             trial[i] = a;
             Q = build_plan(pb, st, trial, ncur, &cq);
             if (!Q) continue;
-            t = time_plan(Q);
-            if (t < best * 0.98) {
+            t = time_plan(Q, &sp_t);
+            if (clearly_faster(P, &best, sp_best, Q, t, sp_t)) {
                 fk_plan_destroy(P);
                 P = Q;
-                best = t;
+                sp_best = sp_t;
                 ncur = Q->nchoice;
                 memcpy(cur, Q->choice, sizeof(int) * ncur);
                 cb = cq;

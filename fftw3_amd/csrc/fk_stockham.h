@@ -661,6 +661,147 @@ stockham(fk_args a)
                                                                smem_raw);
 }
 
+/*
+ * Row-mapped plain pass whose LDS exchange moves the real parts, then the
+ * imaginary parts, through one array of reals: half the LDS per line of
+ * stockham<> (more workgroups resident per CU to hide HBM latency) for
+ * four workgroup barriers per exchange instead of two.  Stage s computes in
+ * registers, writes its outputs' real parts, every thread reads the real
+ * parts of its next-stage inputs, then the same for the imaginary parts.
+ */
+template <class RL, int S, int L, int TPL>
+struct XsGeo {
+    static constexpr int Rs = RL::r[S];
+    static constexpr int Ns = RL::ns(S);
+    static constexpr int NB = L / Rs;
+    static constexpr int BPT = (NB + TPL - 1) / TPL;
+};
+
+template <typename R, class RL, int S, int L, int TPL, int PAD, class LastF>
+__device__ __forceinline__ void run_xs(int u, const cx<R> *__restrict__ tw, R *buf,
+                                       cx<R> (&v)[XsGeo<RL, S, L, TPL>::BPT][XsGeo<RL, S, L, TPL>::Rs],
+                                       LastF &&last)
+{
+    using X = XsGeo<RL, S, L, TPL>;
+    constexpr int Rs = X::Rs, Ns = X::Ns, NB = X::NB, BPT = X::BPT;
+    auto idx = [](int j) { return j + j / PAD; };
+    sfor<0, BPT>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int b = u + i * TPL;
+        if ((NB % TPL == 0) || b < NB) {
+            if constexpr (S > 0) {
+                const int k = b % Ns;
+                const cx<R> *w = tw + RL::twoff(S) + k * (Rs - 1);
+                sfor<1, Rs>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    v[i][r] = mul(v[i][r], w[r - 1]);
+                });
+            }
+            Dft<R, Rs, 1>::run(v[i]);
+        }
+    });
+    auto each_out = [&](auto &&f) {
+        sfor<0, BPT>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const int b = u + i * TPL;
+            if ((NB % TPL == 0) || b < NB) {
+                const int k = b % Ns, base = (b - k) * Rs + k;
+                sfor<0, Rs>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    f(base + r * Ns, v[i][r]);
+                });
+            }
+        });
+    };
+    if constexpr (S == RL::n - 1) {
+        each_out([&](int k, cx<R> x) { last(k, x); });
+    } else {
+        using Y = XsGeo<RL, S + 1, L, TPL>;
+        cx<R> w[Y::BPT][Y::Rs];
+        auto each_in = [&](auto &&f) {
+            sfor<0, Y::BPT>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int b = u + i * TPL;
+                if ((Y::NB % TPL == 0) || b < Y::NB) {
+                    sfor<0, Y::Rs>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
+                        f(b + r * Y::NB, w[i][r]);
+                    });
+                }
+            });
+        };
+        each_out([&](int k, cx<R> x) { buf[idx(k)] = x.x; });
+        __syncthreads();
+        each_in([&](int j, cx<R> &x) { x.x = buf[idx(j)]; });
+        __syncthreads();
+        each_out([&](int k, cx<R> x) { buf[idx(k)] = x.y; });
+        __syncthreads();
+        each_in([&](int j, cx<R> &x) { x.y = buf[idx(j)]; });
+        __syncthreads();
+        run_xs<R, RL, S + 1, L, TPL, PAD>(u, tw, buf, w, last);
+    }
+}
+
+template <typename R, int NL, int PAD, class RL, int TPL>
+__global__ void __launch_bounds__(TPL * NL)
+stockham_xs(fk_args a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int L = RL::L();
+    constexpr int LP = L + L / PAD + 2;
+    const long long tile = remap_tile(a.xcd_remap);
+    const int tid = threadIdx.x, lt = tid / TPL, u = tid % TPL;
+    const long long line = tile * NL + lt;
+    const bool act = line < a.nlines;
+    long long q = act ? line : 0;
+    const long long i0 = q % a.nb[0];
+    q /= a.nb[0];
+    const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
+    const long long bin = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
+    const long long bout = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
+    const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.tw);
+    const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+    cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+    const unsigned fl = a.flags;
+    const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
+    R *buf = reinterpret_cast<R *>(smem_raw) + lt * LP;
+    /* output post-op as in stockham_body: four-step twiddle anchored in the
+       FP64 table every ANCH outputs of a butterfly, scale, swap */
+    constexpr int NS_LAST = RL::ns(RL::n - 1);
+    constexpr int ANCH = sizeof(R) == 8 ? 4 : 16;
+    cx<double> tw_cur = mk(1.0, 0.0), tw_step = mk(1.0, 0.0);
+    if (fl & FK_TW_OUT) tw_step = tw2d(a, (long long)NS_LAST * i0);
+    auto gst = [&](int k, cx<R> v) {
+        if (fl & FK_TW_OUT) {
+            if ((k / NS_LAST) % ANCH == 0) tw_cur = tw2d(a, (long long)k * i0);
+            else tw_cur = mul(tw_cur, tw_step);
+            v = mul(v, mk((R)tw_cur.x, (R)tw_cur.y));
+        }
+        if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        if (act) out[bout + k * a.os] = swo ? swp(v) : v;
+    };
+    using X0 = XsGeo<RL, 0, L, TPL>;
+    cx<R> v[X0::BPT][X0::Rs];
+    sfor<0, X0::BPT>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const int b = u + i * TPL;
+        if ((X0::NB % TPL == 0) || b < X0::NB) {
+            sfor<0, X0::Rs>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                cx<R> x = act ? in[(long long)(b + r * X0::NB) * a.is] : mk((R)0, (R)0);
+                v[i][r] = swi ? swp(x) : x;
+            });
+        }
+    });
+    run_xs<R, RL, 0, L, TPL, PAD>(u, tw, buf, v, gst);
+}
+
+template <typename R, int NL, int PAD, class RL, int TPL>
+constexpr unsigned stockham_xs_lds()
+{
+    return (unsigned)(NL * (RL::L() + RL::L() / PAD + 2) * sizeof(R));
+}
+
 template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT = TPL * NL>
 constexpr unsigned stockham_lds()
 {

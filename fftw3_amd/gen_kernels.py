@@ -24,6 +24,11 @@ CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
 XR_ROWS = 8                   # rows per tile of the cross-row real kernels
+# row-mapped plain passes also offered with the split (real, then
+# imaginary) LDS exchange, where it measured faster than the other
+# variants on a rotated 2048 x 2048 FP64 batch (4096 FP64 and 8192 FP32
+# measured slower: their registers, not their LDS, limit residency)
+XS_SIZES = {("double", 2048)}
 
 
 def factorizations(n, maxr=16):
@@ -181,6 +186,14 @@ def catalog():
                         g16 = geometry(prec, L, rads, mode, io)
                         if g16:
                             entries.append((prec, L, rads, mode, io, g16))
+                    if mode == "ROW" and io == "PLAIN" and (prec, L) in XS_SIZES:
+                        k = L.bit_length() - 1
+                        r8 = [8] * (k // 3) + ([1 << (k % 3)] if k % 3 else [])
+                        gx = geometry(prec, L, r8, mode, io)
+                        if gx:
+                            lp = L + L // gx["pad"] + 2
+                            entries.append((prec, L, r8, mode, io,
+                                            dict(gx, xs=1, lds=lp * gx["nl"] * CSZ[prec] // 2)))
             # half-length kernels for even real transforms (n = 2L)
             if (fam or L <= 64) and L <= 8192:
                 for io in ("R2C", "C2R"):
@@ -401,6 +414,13 @@ def main():
             prec, L, rads, mode, io, g, var = e
             targs, rl = cxx_entry(e[:6])
             radarr = ",".join(map(str, rads))
+            if g.get("xs"):
+                xargs = "%s,%d,%d,%s,%d" % (prec, g["nl"], g["pad"], rl, g["tpl"])
+                lines.append(
+                    '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_xs_lds<%s>(), '
+                    '%s::twsize(), (const void *)&fk::stockham_xs<%s>, %d},'
+                    % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, xargs, rl, xargs, var))
+                continue
             lines.append(
                 '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '
                 '%s::twsize(), (const void *)&fk::stockham<%s>, %d},'
