@@ -677,14 +677,13 @@ struct XsGeo {
     static constexpr int BPT = (NB + TPL - 1) / TPL;
 };
 
-template <typename R, class RL, int S, int L, int TPL, int PAD, class LastF>
-__device__ __forceinline__ void run_xs(int u, const cx<R> *__restrict__ tw, R *buf,
+template <typename R, class RL, int S, int L, int TPL, class PosF, class LastF>
+__device__ __forceinline__ void run_xs(int u, const cx<R> *__restrict__ tw, R *buf, PosF &&idx,
                                        cx<R> (&v)[XsGeo<RL, S, L, TPL>::BPT][XsGeo<RL, S, L, TPL>::Rs],
                                        LastF &&last)
 {
     using X = XsGeo<RL, S, L, TPL>;
     constexpr int Rs = X::Rs, Ns = X::Ns, NB = X::NB, BPT = X::BPT;
-    auto idx = [](int j) { return j + j / PAD; };
     sfor<0, BPT>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         const int b = u + i * TPL;
@@ -738,11 +737,11 @@ __device__ __forceinline__ void run_xs(int u, const cx<R> *__restrict__ tw, R *b
         __syncthreads();
         each_in([&](int j, cx<R> &x) { x.y = buf[idx(j)]; });
         __syncthreads();
-        run_xs<R, RL, S + 1, L, TPL, PAD>(u, tw, buf, w, last);
+        run_xs<R, RL, S + 1, L, TPL>(u, tw, buf, idx, w, last);
     }
 }
 
-template <typename R, int NL, int PAD, class RL, int TPL>
+template <typename R, int NL, int MODE, int PAD, class RL, int TPL>
 __global__ void __launch_bounds__(TPL * NL)
 stockham_xs(fk_args a)
 {
@@ -750,7 +749,12 @@ stockham_xs(fk_args a)
     constexpr int L = RL::L();
     constexpr int LP = L + L / PAD + 2;
     const long long tile = remap_tile(a.xcd_remap);
-    const int tid = threadIdx.x, lt = tid / TPL, u = tid % TPL;
+    /* row mapping: a line's threads are consecutive, its reals one padded
+       LDS row; column mapping: consecutive threads are consecutive lines
+       (unit-stride batch), LDS laid out [element][line] */
+    const int tid = threadIdx.x;
+    const int lt = MODE == FK_ROW ? tid / TPL : tid % NL;
+    const int u = MODE == FK_ROW ? tid % TPL : tid / NL;
     const long long line = tile * NL + lt;
     const bool act = line < a.nlines;
     long long q = act ? line : 0;
@@ -764,7 +768,8 @@ stockham_xs(fk_args a)
     cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
     const unsigned fl = a.flags;
     const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
-    R *buf = reinterpret_cast<R *>(smem_raw) + lt * LP;
+    R *buf = reinterpret_cast<R *>(smem_raw) + (MODE == FK_ROW ? lt * LP : lt);
+    auto idx = [](int j) { return MODE == FK_ROW ? j + j / PAD : j * NL; };
     /* output post-op as in stockham_body: four-step twiddle anchored in the
        FP64 table every ANCH outputs of a butterfly, scale, swap */
     constexpr int NS_LAST = RL::ns(RL::n - 1);
@@ -793,13 +798,13 @@ stockham_xs(fk_args a)
             });
         }
     });
-    run_xs<R, RL, 0, L, TPL, PAD>(u, tw, buf, v, gst);
+    run_xs<R, RL, 0, L, TPL>(u, tw, buf, idx, v, gst);
 }
 
-template <typename R, int NL, int PAD, class RL, int TPL>
+template <typename R, int NL, int MODE, int PAD, class RL, int TPL>
 constexpr unsigned stockham_xs_lds()
 {
-    return (unsigned)(NL * (RL::L() + RL::L() / PAD + 2) * sizeof(R));
+    return (unsigned)(NL * (MODE == FK_ROW ? RL::L() + RL::L() / PAD + 2 : RL::L()) * sizeof(R));
 }
 
 template <typename R, int NL, int MODE, int IO, int PAD, class RL, int TPL, int NT = TPL * NL>

@@ -23,7 +23,7 @@ LDS_TARGET = 64 * 1024        # keep >= 2 workgroups per CU when possible
 CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
-XR_ROWS = 8                   # rows per tile of the cross-row real kernels
+XR_ROWS = (8,)                # rows per tile of the cross-row real kernels
 # row-mapped plain passes also offered with the split (real, then
 # imaginary) LDS exchange, where it measured faster than the other
 # variants on a rotated 2048 x 2048 FP64 batch (4096 FP64 and 8192 FP32
@@ -215,10 +215,12 @@ def catalog():
             # its tiles hold 8x more lines and read longer runs
             if (prec, L) in (("float", 512), ("float", 1024), ("double", 512)):
                 g = geometry(prec, L, rads, "ROW", "R2C")
-                if g:
+                for rows in XR_ROWS:
+                    if not g:
+                        break
                     lp = L + L // g["pad"] + 2
-                    gx = dict(g, nl=XR_ROWS, lds=lp * XR_ROWS * CSZ[prec] + 8 * XR_ROWS)
-                    if gx["lds"] <= LDS_MAX and gx["tpl"] * XR_ROWS <= 1024:
+                    gx = dict(g, nl=rows, lds=lp * rows * CSZ[prec] + 8 * rows)
+                    if gx["lds"] <= LDS_MAX and gx["tpl"] * rows <= 1024:
                         entries.append((prec, L, rads, "ROW", "R2C_XR", gx))
                         entries.append((prec, L, rads, "ROW", "C2R_XR", gx))
             # full-length real kernels for odd lengths (no half-length form)
@@ -415,7 +417,7 @@ def main():
             targs, rl = cxx_entry(e[:6])
             radarr = ",".join(map(str, rads))
             if g.get("xs"):
-                xargs = "%s,%d,%d,%s,%d" % (prec, g["nl"], g["pad"], rl, g["tpl"])
+                xargs = "%s,%d,FK_%s,%d,%s,%d" % (prec, g["nl"], mode, g["pad"], rl, g["tpl"])
                 lines.append(
                     '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_xs_lds<%s>(), '
                     '%s::twsize(), (const void *)&fk::stockham_xs<%s>, %d},'
