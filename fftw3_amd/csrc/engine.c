@@ -1203,7 +1203,10 @@ static const fk_kernel *xr_kernel(ctx *c, const fk_problem *pb, int io, fk_dim *
     const fk_kernel *k;
     fk_dim e[FK_MAXRANK * 2];
     if (r < 2 || nl % 2 || pb->in == pb->out || c->r16) return 0;
-    k = find_kernel(c->P->prec, nl / 2, FK_ROW, io);
+    /* rows per tile: 8 by default, 4 (half the LDS, a 2x longer column
+       pass) as the tuner's alternative */
+    if (!find_kernel(c->P->prec, nl / 2, FK_ROW, io)) return 0;
+    k = pick_variant(c, c->P->prec, nl / 2, FK_ROW, io);
     if (!k) return 0;
     n0 = pb->dims[r - 2].n;
     if (n0 % k->nl || n0 / k->nl < 2) return 0;

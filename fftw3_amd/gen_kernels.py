@@ -23,7 +23,7 @@ LDS_TARGET = 64 * 1024        # keep >= 2 workgroups per CU when possible
 CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
-XR_ROWS = (8,)                # rows per tile of the cross-row real kernels
+XR_ROWS = (8, 4)              # rows per tile of the cross-row real kernels
 # row-mapped plain passes also offered with the split (real, then
 # imaginary) LDS exchange, where it measured faster than the other
 # variants on a rotated 2048 x 2048 FP64 batch (4096 FP64 and 8192 FP32
@@ -213,6 +213,7 @@ def catalog():
             # dimension's DFT across the 8 rows of a tile (2-D and higher
             # real transforms): the remaining column pass is 8x shorter, so
             # its tiles hold 8x more lines and read longer runs
+            # (8 rows first; 4 rows, half the LDS, as the tuner's alternative)
             if (prec, L) in (("float", 512), ("float", 1024), ("double", 512)):
                 g = geometry(prec, L, rads, "ROW", "R2C")
                 for rows in XR_ROWS:
