@@ -50,6 +50,16 @@ struct RadList {
     static constexpr int twsize() { return twoff(n); }
 };
 
+/* Element index b + OFF of a line, b a multiple of ALIGN.  The padded row
+   layout keeps the compile-time part OFF out of the per-element address
+   arithmetic (it becomes the LDS instruction's offset); every other
+   accessor takes the plain index through the conversion. */
+template <int OFF, int ALIGN = 1>
+struct At {
+    int b;
+    __device__ __forceinline__ constexpr operator int() const { return b + OFF; }
+};
+
 /* XL = 1 adds one scratch line for threads beyond TPL*NL (a persistent
    workgroup running a geometry smaller than itself). */
 template <typename R, int L, int TPL, int NL, int MODE, int PAD, int XL = 0>
@@ -61,9 +71,18 @@ struct Geo {
     __device__ __forceinline__ static int idx(int lt, int j)
     {
         if constexpr (MODE == FK_ROW)
-            return lt * LP + j + j / PAD;
+            return lt * LP + j + (int)((unsigned)j / (unsigned)PAD);
         else
             return j * NLP + lt;
+    }
+    /* (b + OFF) / PAD = b / PAD + OFF / PAD when PAD divides OFF or b */
+    template <int OFF, int ALIGN>
+    __device__ __forceinline__ static int idx(int lt, At<OFF, ALIGN> j)
+    {
+        if constexpr (MODE == FK_ROW && (OFF % PAD == 0 || ALIGN % PAD == 0))
+            return lt * LP + j.b + (int)((unsigned)j.b / (unsigned)PAD) + (OFF + OFF / PAD);
+        else
+            return idx(lt, (int)j);
     }
     static constexpr unsigned lds_elems()
     {
@@ -112,7 +131,7 @@ __device__ __forceinline__ void stage(int u, const cx<R> *__restrict__ tw,
         if ((NB % TPL == 0) || b < NB) {
             sfor<0, Rs>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                v[i][r] = ld(b + r * NB);
+                v[i][r] = ld(At<r * NB>{b});
             });
         }
     });
@@ -121,7 +140,9 @@ __device__ __forceinline__ void stage(int u, const cx<R> *__restrict__ tw,
         constexpr int i = decltype(ic)::value;
         const int b = u + i * TPL;
         if ((NB % TPL == 0) || b < NB) {
-            const int k = b % Ns;
+            /* the last stage has Ns = NB, so b < Ns there: its outputs are
+               b + r*Ns, a compile-time offset from the thread's index */
+            const int k = S == RL::n - 1 ? b : b % Ns;
             if constexpr (S > 0) {
                 const cx<R> *w = tw + RL::twoff(S) + k * (Rs - 1);
                 sfor<1, Rs>([&](auto rc) {
@@ -131,9 +152,10 @@ __device__ __forceinline__ void stage(int u, const cx<R> *__restrict__ tw,
             }
             Dft<R, Rs, 1>::run(v[i]);
             const int base = (b - k) * Rs + k;
+            /* stage 0 (Ns = 1) writes Rs consecutive outputs from b*Rs */
             sfor<0, Rs>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
-                st(base + r * Ns, v[i][r]);
+                st(At<r * Ns, (S == 0 ? Rs : 1)>{base}, v[i][r]);
             });
         }
     });
@@ -148,8 +170,8 @@ __device__ __forceinline__ void run_stages(int u, const cx<R> *tw, LdsF &&lds,
                                            FirstF &&first, LastF &&last)
 {
     constexpr int n = RL::n;
-    auto lds_ld = [&](int j) { return lds(j); };
-    auto lds_st = [&](int k, cx<R> v) { lds(k) = v; };
+    auto lds_ld = [&](auto j) { return lds(j); };
+    auto lds_st = [&](auto k, cx<R> v) { lds(k) = v; };
     if constexpr (S == 0 && n == 1) {
         stage<R, RL, 0, L, TPL, FIRST_LDS, LAST_LDS, SY>(u, tw, first, last);
     } else if constexpr (S == 0) {
@@ -214,8 +236,11 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
     const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.tw);
     const unsigned fl = a.flags;
     const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
+    /* output scale: multiplying by 1 when unscaled is exact and costs less
+       than selecting between scaled and unscaled values */
+    const R sc = (fl & FK_SCALE) ? (R)a.scale : (R)1;
 
-    auto lds_at = [&](int j) -> cx<R> & {
+    auto lds_at = [&](auto j) -> cx<R> & {
         if constexpr (NT == NR) return lds[G::idx(lt, j)];
         else return real ? lds[G::idx(lt, j)] : trash[tid & 63];
     };
@@ -238,24 +263,30 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             else tw_cur = mul(tw_cur, tw_step);
             v = mul(v, mk((R)tw_cur.x, (R)tw_cur.y));
         }
-        if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        v.x *= sc; v.y *= sc;
         return swo ? swp(v) : v;
     };
 
     if constexpr (IO == FK_IO_PLAIN || IO == FK_IO_TOUT) {
-        const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in) + bin;
+        /* element j of the thread's line sits (j - u) strides from the
+           thread's own element u; the stages pass j = u + (compile-time
+           offset), so each access is one uniform product and one 64-bit
+           add.  Inactive lanes read line 0 (in bounds) and store nothing. */
+        const long long is = a.is, os = a.os;
+        const cx<R> *inu = reinterpret_cast<const cx<R> *>(a.in) + bin + u * is;
         cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
         auto gld = [&](int j) -> cx<R> {
-            cx<R> v = act ? in[j * a.is] : mk((R)0, (R)0);
+            cx<R> v = inu[(long long)(j - u) * is];
             return swi ? swp(v) : v;
         };
         if constexpr (IO == FK_IO_PLAIN) {
+            cx<R> *outu = out + bout + u * os;
             auto gst = [&](int k, cx<R> v) {
-                if (act) out[bout + k * a.os] = post(k, v);
+                if (act) outu[(long long)(k - u) * os] = post(k, v);
             };
             run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
         } else {
-            auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+            auto lst = [&](auto k, cx<R> v) { lds_at(k) = v; };
             /* output offsets may be negative (negative strides): inactive
                lines are marked with the one value no offset takes */
             constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
@@ -270,7 +301,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 const long long ob = lbase[l2];
                 if (ob != NONE) {
                     cx<R> v = lds[G::idx(l2, k)];
-                    if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+                    v.x *= sc; v.y *= sc;
                     out[ob + k * a.os] = swo ? swp(v) : v;
                 }
             }
@@ -285,7 +316,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         };
         auto gst = [&](int k, cx<R> v) {
             if (act && 2 * k <= L) {
-                if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+                v.x *= sc; v.y *= sc;
                 out[bout + (long long)k * a.os] = v;
             }
         };
@@ -311,7 +342,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         auto gst = [&](int k, cx<R> v) {
             if (!act) return;
             R y = v.y;
-            if (fl & FK_SCALE) y *= (R)a.scale;
+            y *= sc;
             outr[(long long)k * a.os] = y;
         };
         run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, gld, gst);
@@ -335,7 +366,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             if (!act) return;
             const cx<R> c = mul(v, rt[4 * p]);
             R ye = (R)2 * c.x, yo = (R)-2 * c.y;
-            if (fl & FK_SCALE) { ye *= (R)a.scale; yo *= (R)a.scale; }
+            ye *= sc; yo *= sc;
             outr[(2LL * p) * a.os] = ye;
             outr[(N2 - 1 - 2LL * p) * a.os] = sn ? -yo : yo;
         };
@@ -392,7 +423,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             if (fl & FK_REAL_UNIT) return reinterpret_cast<const cx<R> *>(inr)[j];
             return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
         };
-        auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+        auto lst = [&](auto k, cx<R> v) { lds_at(k) = v; };
         run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* bins k = u + i*TPL in groups of 8 (compile-time count): the
            group's twiddle loads are in flight together */
@@ -425,7 +456,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     /* O = (Z[k] - conj Z[L-k]) / (2i) */
                     const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
                     cx<R> X = mk(er + orr * w[i].x - oi * w[i].y, ei + orr * w[i].y + oi * w[i].x);
-                    if (fl & FK_SCALE) { X.x *= (R)a.scale; X.y *= (R)a.scale; }
+                    X.x *= sc; X.y *= sc;
                     if constexpr (XR) {
                         xs[g * GK + i] = X;
                     } else if (act && rm == FK_RM_NONE) {
@@ -597,7 +628,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         };
         auto gst = [&](int k, cx<R> v) {
             /* v = swap(z_k) */
-            if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+            v.x *= sc; v.y *= sc;
             if (!act) return;
             if (rm == FK_RM_DCT3 || rm == FK_RM_DST3) {
                 /* y_2i = v_i (i < L), y_{2i+1} = v_{n-1-i}: v_m goes to 2m
@@ -635,13 +666,13 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             if (swi) v = swp(v);
             return mul(v, chirp[j]);
         };
-        auto lst = [&](int k, cx<R> v) { lds_at(k) = v; };
+        auto lst = [&](auto k, cx<R> v) { lds_at(k) = v; };
         run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         auto mid = [&](int j) -> cx<R> { return swp(mul(lds_at(j), Bt[j])); };
         auto gst = [&](int k, cx<R> v) {
             if (k < n && act) {
                 cx<R> y = mul(swp(v), chirp[k]);
-                if (fl & FK_SCALE) { y.x *= (R)a.scale; y.y *= (R)a.scale; }
+                y.x *= sc; y.y *= sc;
                 out[bout + k * a.os] = swo ? swp(y) : y;
             }
         };
@@ -768,6 +799,9 @@ stockham_xs(fk_args a)
     cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
     const unsigned fl = a.flags;
     const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
+    /* output scale: multiplying by 1 when unscaled is exact and costs less
+       than selecting between scaled and unscaled values */
+    const R sc = (fl & FK_SCALE) ? (R)a.scale : (R)1;
     R *buf = reinterpret_cast<R *>(smem_raw) + (MODE == FK_ROW ? lt * LP : lt);
     auto idx = [](int j) { return MODE == FK_ROW ? j + j / PAD : j * NL; };
     /* output post-op as in stockham_body: four-step twiddle anchored in the
@@ -782,7 +816,7 @@ stockham_xs(fk_args a)
             else tw_cur = mul(tw_cur, tw_step);
             v = mul(v, mk((R)tw_cur.x, (R)tw_cur.y));
         }
-        if (fl & FK_SCALE) { v.x *= (R)a.scale; v.y *= (R)a.scale; }
+        v.x *= sc; v.y *= sc;
         if (act) out[bout + k * a.os] = swo ? swp(v) : v;
     };
     using X0 = XsGeo<RL, 0, L, TPL>;
