@@ -465,14 +465,14 @@ def catalog():
     """The kernel catalog: one dict per entry (fftw_amd_catalog_entry)."""
     L = lib()
     out = []
-    info = (ctypes.c_int * 21)()
+    info = (ctypes.c_int * 22)()
     for i in range(L.fftw_amd_catalog_size()):
         if L.fftw_amd_catalog_entry(i, info):
             raise RuntimeError("catalog entry %d" % i)
         v = list(info)
         out.append(dict(index=i, prec="double" if v[0] == 0 else "float", L=v[1],
                         mode="row" if v[2] == 0 else "col", io=IO_NAMES[v[3]], tpl=v[4], nl=v[5],
-                        variant=v[6], rad=v[8:8 + v[7]], lds=v[20]))
+                        variant=v[6], rad=v[8:8 + v[7]], lds=v[20], loop=v[21]))
     return out
 
 

@@ -892,6 +892,7 @@ int fftw_amd_catalog_entry(int i, int *info)
     info[7] = k->nrad;
     for (j = 0; j < 12; ++j) info[8 + j] = j < k->nrad ? k->rad[j] : 0;
     info[20] = (int)k->lds;
+    info[21] = k->loop;
     return 0;
 }
 int fftw_amd_force_kernel(int i) { return fk_force_kernel(i); }

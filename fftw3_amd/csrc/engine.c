@@ -424,6 +424,23 @@ int fk_force_kernel(int i)
     return 0;
 }
 
+/* Workgroups of kernel fn (block threads, lds bytes) that the current
+   device holds at once over all its CUs: the grid of the kernels whose
+   workgroups run several tiles.  0 when the runtime cannot say (the caller
+   keeps one workgroup per tile). */
+static unsigned resident_wgs(const void *fn, unsigned block, unsigned lds)
+{
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, (int)block, lds) != hipSuccess ||
+        ncu <= 0 || per <= 0) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (unsigned)ncu * (unsigned)per;
+}
+
 static fk_pass *new_pass(ctx *c)
 {
     fk_plan *P = c->P;
@@ -560,6 +577,10 @@ static void emit_loops(ctx *c, const fk_pass *proto, int nb, const fk_dim *b0, s
         if (p->type == FK_PASS_STOCKHAM) {
             fill_batch3(p->a.nb, p->a.ids, p->a.ods, &p->a.nlines, nb < 3 ? nb : 3, b);
             p->grid = (unsigned)((p->a.nlines + p->k->nl - 1) / p->k->nl);
+            if (p->k->loop) {
+                unsigned cap = resident_wgs(p->k->fn, p->block, p->lds);
+                if (cap && cap < p->grid) p->grid = cap;
+            }
             if (p->a.nlines == 0) --c->P->npass;
         } else {
             fill_batch3(p->e.nb, p->e.ids, p->e.ods, &p->e.nlines, nb < 3 ? nb : 3, b);
@@ -588,6 +609,27 @@ static void emit_stockham(ctx *c, const fk_kernel *k, long long is, long long os
                           unsigned flags, size_t isz, size_t osz)
 {
     fk_pass proto;
+    if (k->loop && (flags & FK_TW_OUT)) {
+        /* the pipelined kernels have no four-step output twiddle: such
+           passes run the one-tile-per-workgroup kernel of the same
+           geometry */
+        const fk_kernel *vs[16];
+        int n = find_variants(k->prec, k->L, k->mode, k->io, vs, 16), i, j, same;
+        for (i = 0; i < n; ++i) {
+            if (vs[i]->loop || vs[i]->tpl != k->tpl || vs[i]->nl != k->nl || vs[i]->nrad != k->nrad)
+                continue;
+            for (same = 1, j = 0; j < k->nrad; ++j) same &= vs[i]->rad[j] == k->rad[j];
+            if (same) {
+                k = vs[i];
+                break;
+            }
+        }
+        if (k->loop) {
+            c->err = 1;
+            set_err("no twiddled kernel for L=%d", k->L);
+            return;
+        }
+    }
     memset(&proto, 0, sizeof(proto));
     proto.type = FK_PASS_STOCKHAM;
     proto.k = k;
@@ -2800,11 +2842,11 @@ int fk_plan_pass_info(const fk_plan *p, int i, char *buf, int len)
             snprintf(rad + strlen(rad), sizeof(rad) - strlen(rad), "%s%d", j ? "x" : "",
                      ps->k->rad[j]);
         return snprintf(buf, len,
-                        "%s %s L=%d rad=%s %s %s tpl=%d nl=%d lines=%lld grid=%u lds=%u "
+                        "%s %s L=%d rad=%s %s %s tpl=%d nl=%d%s lines=%lld grid=%u lds=%u "
                         "is=%lld os=%lld flags=%u src=%s dst=%s",
                         ps->k->variant >= 100 ? "mfma16" : "stockham", ps->k->prec ? "f32" : "f64", ps->k->L, rad,
                         ps->k->mode == FK_ROW ? "row" : "col", io_name(ps->k->io),
-                        ps->k->tpl, ps->k->nl, ps->a.nlines, ps->grid, ps->lds, ps->a.is,
+                        ps->k->tpl, ps->k->nl, ps->k->loop ? " pipelined" : "", ps->a.nlines, ps->grid, ps->lds, ps->a.is,
                         ps->a.os, ps->a.flags, buf_name(ps->ibuf), buf_name(ps->obuf));
     }
     return snprintf(buf, len, "elementwise op=%d n=%lld lines=%lld src=%s dst=%s", ps->e.op,

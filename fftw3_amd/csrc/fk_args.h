@@ -95,6 +95,8 @@ typedef struct {
     int twsz;          /* complex entries in the per-stage twiddle table */
     const void *fn;    /* kernel host stub for hipLaunchKernel */
     int variant;       /* 0 = default geometry; >0 autotuner alternatives */
+    int loop;          /* 1: each workgroup runs several tiles (grid sized to
+                          the workgroups the device holds at once) */
 } fk_kernel;
 
 /* elementwise passes around the Stockham kernels (multi-pass paths) */

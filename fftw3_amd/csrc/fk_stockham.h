@@ -424,12 +424,33 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
         };
         auto lst = [&](auto k, cx<R> v) { lds_at(k) = v; };
-        run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* bins k = u + i*TPL in groups of 8 (compile-time count): the
            group's twiddle loads are in flight together */
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
         constexpr int GK = 8;
         constexpr bool XR = IO == FK_IO_R2C_XR;
+        /* few bins per thread: their twiddles are loaded before the stages,
+           so the load latency hides behind the FFT instead of following it */
+        constexpr bool PRE = NK <= 20;
+        cx<R> wpre[PRE ? NK : 1];
+        if constexpr (PRE) {
+            sfor<0, NK>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + i * TPL;
+                wpre[i] = wk[k <= L ? k : L];
+            });
+        }
+        /* cross-row form: twiddles of the first column stage, W_n0^(n1*p)
+           (n1 = the tile's line-group index, uniform over the workgroup) */
+        cx<R> wp[XR ? NL : 1];
+        if constexpr (XR) {
+            const long long n1 = tile % a.nb[1];
+            sfor<0, NL>([&](auto pc) {
+                constexpr int pp = decltype(pc)::value;
+                wp[pp] = pp == 0 ? mk((R)1, (R)0) : tw2<R>(a, n1 * pp);
+            });
+        }
+        run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* cross-row form: the bins stay in registers until every bin of the
            line has been formed, then replace Z in the line's LDS row */
         cx<R> xs[XR ? NK : 1];
@@ -444,7 +465,8 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             sfor<0, cnt>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const int k = u + (g * GK + i) * TPL;
-                if (k <= L) w[i] = wk[k];
+                if constexpr (PRE) w[i] = wpre[g * GK + i];
+                else if (k <= L) w[i] = wk[k];
             });
             sfor<0, cnt>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
@@ -506,12 +528,6 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                planner sets nb[0] = NL, so a tile is one n1): for every bin k
                y_p = W_n0^(n1*p) sum_n2 X_n2[k] W_NL^(n2*p), n0 = a.tN,
                stored to the tile's output line p */
-            const long long n1 = tile % a.nb[1];
-            cx<R> wp[NL];
-            sfor<0, NL>([&](auto pc) {
-                constexpr int pp = decltype(pc)::value;
-                wp[pp] = pp == 0 ? mk((R)1, (R)0) : tw2<R>(a, n1 * pp);
-            });
             for (int k = tid; k <= L; k += NT) {
                 cx<R> v[NL];
                 sfor<0, NL>([&](auto qc) {
@@ -690,6 +706,112 @@ stockham(fk_args a)
     using SY = typename std::conditional<wave_lines, SyncWave, SyncAll>::type;
     stockham_body<R, NL, MODE, IO, PAD, RL, TPL, TPL * NL, SY>(a, remap_tile(a.xcd_remap),
                                                                smem_raw);
+}
+
+template <class T>
+struct AtOff;
+template <int OFF, int ALIGN>
+struct AtOff<At<OFF, ALIGN>> {
+    static constexpr int v = OFF;
+};
+
+/*
+ * Row-mapped plain pass with one stage-0 butterfly per thread (L / R0 =
+ * TPL) whose workgroups each run several tiles: the grid holds the
+ * workgroups the device keeps resident, a workgroup steps through tiles
+ * blockIdx.x + k*gridDim.x, and the next tile's stage-0 inputs are loaded
+ * into registers while the current tile runs its stages, so HBM reads
+ * overlap the LDS stages that otherwise follow each tile's loads.  For
+ * long lines (one or a few per workgroup) the whole batch is only a couple
+ * of workgroup rounds, which stockham<> runs with load, compute and store
+ * phases in lock-step.  Same arithmetic as stockham<>: identical output.
+ */
+template <typename R, int NL, int PAD, class RL, int TPL>
+__global__ void __launch_bounds__(TPL * NL, 4)
+stockham_pipe(fk_args a)
+{
+    constexpr int L = RL::L(), R0 = RL::r[0], NB0 = L / R0;
+    static_assert(NB0 == TPL, "one stage-0 butterfly per thread");
+    using G = Geo<R, L, TPL, NL, FK_ROW, PAD>;
+    constexpr bool wave_lines = TPL <= 64 && 64 % TPL == 0;
+    using SY = typename std::conditional<wave_lines, SyncWave, SyncAll>::type;
+    extern __shared__ unsigned char smem_raw[];
+    cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
+    const int tid = threadIdx.x, lt = tid / TPL, u = tid % TPL;
+    const long long ntiles = (a.nlines + NL - 1) / NL, step = gridDim.x;
+    const unsigned fl = a.flags;
+    const bool swi = fl & FK_SWAP_IN, swo = fl & FK_SWAP_OUT;
+    const R sc = (fl & FK_SCALE) ? (R)a.scale : (R)1;
+    const long long is = a.is, os = a.os;
+    const cx<R> *tw = reinterpret_cast<const cx<R> *>(a.tw);
+    const cx<R> *in = reinterpret_cast<const cx<R> *>(a.in);
+    cx<R> *out = reinterpret_cast<cx<R> *>(a.out);
+    /* line of tile t for this thread: input and output offsets, computed
+       once per tile when its inputs are fetched; lanes past the last line
+       use line 0 (in bounds) and store nothing */
+    auto line_of = [&](long long t, long long &bin, long long &bout) -> bool {
+        const long long line = t * NL + lt;
+        const bool act = line < a.nlines;
+        long long q = act ? line : 0;
+        const long long i0 = q % a.nb[0];
+        q /= a.nb[0];
+        const long long i1 = q % a.nb[1], i2 = q / a.nb[1];
+        bin = i0 * a.ids[0] + i1 * a.ids[1] + i2 * a.ids[2];
+        bout = i0 * a.ods[0] + i1 * a.ods[1] + i2 * a.ods[2];
+        return act;
+    };
+    long long t = blockIdx.x;
+    if (t >= ntiles) return;
+    cx<R> cur[R0], nxt[R0];
+    /* stage-0 inputs of the thread's butterfly */
+    auto fetch = [&](long long bin, cx<R> *v) {
+        const cx<R> *p = in + bin + u * is;
+        sfor<0, R0>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            const cx<R> x = p[(long long)(r * NB0) * is];
+            v[r] = swi ? swp(x) : x;
+        });
+    };
+    long long bin, bout;
+    bool act = line_of(t, bin, bout);
+    fetch(bin, cur);
+    auto lds_at = [&](auto j) -> cx<R> & { return lds[G::idx(lt, j)]; };
+    /* output post-op of stockham<> without the four-step twiddle (the
+       planner runs passes that need it with stockham<>): scale, swap */
+    auto post = [&](cx<R> v) -> cx<R> {
+        v.x *= sc;
+        v.y *= sc;
+        return swo ? swp(v) : v;
+    };
+    for (;;) {
+        const long long tn = t + step;
+        const bool more = tn < ntiles;
+        long long bin_n = 0, bout_n = 0;
+        bool act_n = false;
+        if (more) {
+            act_n = line_of(tn, bin_n, bout_n);
+            fetch(bin_n, nxt);
+        }
+        cx<R> *outu = out + bout + u * os;
+        auto first = [&](auto j) -> cx<R> {
+            constexpr int r = AtOff<typename std::decay<decltype(j)>::type>::v / NB0;
+            return cur[r];
+        };
+        auto gst = [&](auto k, cx<R> v) {
+            if (NL == 1 || act) outu[(long long)((int)k - u) * os] = post(v);
+        };
+        run_stages<R, RL, 0, L, TPL, false, false, SY>(u, tw, lds_at, first, gst);
+        if (!more) break;
+        sfor<0, R0>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            cur[r] = nxt[r];
+        });
+        t = tn;
+        bout = bout_n;
+        act = act_n;
+        /* the next tile's stage-0 LDS writes follow every read of this one */
+        __syncthreads();
+    }
 }
 
 /*

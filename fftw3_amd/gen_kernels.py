@@ -24,6 +24,14 @@ CSZ = {"double": 16, "float": 8}
 PREC = {"double": 0, "float": 1}
 RADICES = [16, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2]
 XR_ROWS = (8, 4)              # rows per tile of the cross-row real kernels
+# row-mapped plain passes of long lines also offered pipelined: each
+# workgroup runs several tiles and loads the next tile's first-stage inputs
+# during the current tile's stages (needs one first-stage butterfly per
+# thread).  The default for FP64 2048 and 4096 (measured 25% and 8% faster
+# on batches beyond the memory-side cache), a tuner alternative for FP32
+# 4096; FP32 8192 measured slower.
+PIPE_DEFAULT = {("double", 2048), ("double", 4096)}
+PIPE_SIZES = PIPE_DEFAULT | {("float", 4096)}
 # row-mapped plain passes also offered with the split (real, then
 # imaginary) LDS exchange, where it measured faster than the other
 # variants on a rotated 2048 x 2048 FP64 batch (4096 FP64 and 8192 FP32
@@ -180,8 +188,14 @@ def catalog():
                     r0 = [8] * ((L.bit_length() - 1) // 3) + ([4] if L == 2048 else [])
                 g = geometry(prec, L, r0, mode, io)
                 if g:
+                    pipe = mode == "ROW" and io == "PLAIN" and (prec, L) in PIPE_SIZES \
+                        and L // r0[0] == g["tpl"]
+                    if pipe and (prec, L) in PIPE_DEFAULT:
+                        entries.append((prec, L, r0, mode, io, dict(g, pipe=1)))
                     entries.append((prec, L, r0, mode, io, g))
                     entries += variants(prec, L, r0, mode, io, g)
+                    if pipe and (prec, L) not in PIPE_DEFAULT:
+                        entries.append((prec, L, r0, mode, io, dict(g, pipe=1)))
                     if r0 is not rads:
                         g16 = geometry(prec, L, rads, mode, io)
                         if g16:
@@ -423,6 +437,13 @@ def main():
                     '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_xs_lds<%s>(), '
                     '%s::twsize(), (const void *)&fk::stockham_xs<%s>, %d},'
                     % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, xargs, rl, xargs, var))
+                continue
+            if g.get("pipe"):
+                pargs = "%s,%d,%d,%s,%d" % (prec, g["nl"], g["pad"], rl, g["tpl"])
+                lines.append(
+                    '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '
+                    '%s::twsize(), (const void *)&fk::stockham_pipe<%s>, %d, 1},'
+                    % (PREC[prec], L, len(rads), radarr, g["tpl"], g["nl"], mode, io, targs, rl, pargs, var))
                 continue
             lines.append(
                 '        {%d, %d, %d, {%s}, %d, %d, FK_%s, FK_IO_%s, fk::stockham_lds<%s>(), '

@@ -91,14 +91,17 @@ double fftw_amd_plan_time(void *plan, void *in, void *out, int iters, double *pe
 int fftw_amd_execute_noalloc(const void *plan, void *in, void *out);
 
 /* Kernel catalog (the compiled Stockham kernels the planner chooses from).
-   fftw_amd_catalog_entry fills info[21]: precision (0 double, 1 float),
+   fftw_amd_catalog_entry fills info[22]: precision (0 double, 1 float),
    length L, mapping (0 row, 1 column), IO kind (0 plain, 1 transposed
    store, 2 r2c, 3 c2r, 4 Bluestein, 5 r2c with a real-to-real kind, 6 c2r
    with a real-to-real kind, 7 DCT-IV, 8 full-length r2c, 9 full-length
    c2r, 10 r2c with the first radix-(lines per workgroup) stage of the next
    dimension across the workgroup's rows, 11 c2r with the last inverse such
    stage), threads per line, lines per workgroup, variant (0 = the default
-   geometry), number of radices, the radices (12 slots), dynamic LDS bytes.
+   geometry), number of radices, the radices (12 slots), dynamic LDS bytes,
+   1 when each workgroup runs several tiles, loading the next tile's
+   inputs during the current tile's stages (grid sized to the resident
+   workgroups).
    Returns 0, or -1 for an index out of range. */
 int fftw_amd_catalog_size(void);
 int fftw_amd_catalog_entry(int i, int *info);
