@@ -2353,6 +2353,87 @@ static double now_s(void)
 #define TUNE_MIN_MS 1.0
 #define TUNE_LIMIT_S 2.0
 static double fk_plan_time_run(fk_plan *p, int iter);
+/* The memory-side cache (256 MB on MI355X) would serve a plan whose arrays
+   fit in it from the cache on every repeat of the timing loop, which is not
+   how the passes meet data in a stream of transforms.  Such a plan is timed
+   instead on a rotation of device copies of its arrays (at least twice the
+   cache in all), so FFTW_MEASURE ranks candidates on HBM-resident data. */
+#define TUNE_CACHE_BYTES ((size_t)256 << 20)
+#define TUNE_ROT_MAX 16
+typedef struct {
+    int n;
+    void *buf[2 * TUNE_ROT_MAX];   /* copy i: buf[2i] input, buf[2i + 1] output */
+    void *own[2 * TUNE_ROT_MAX];   /* what tune_rot_begin allocated */
+} tune_rot;
+
+/* bytes spanned by an array of element size esz addressed by the
+   (n, stride) pairs (stride of pick 0: input, 1: output); 0 when a stride
+   is negative */
+static size_t array_extent(const fk_dim *d, int nd, const fk_dim *v, int nv, int pick, size_t esz)
+{
+    long long span = 0;
+    int i;
+    for (i = 0; i < nd + nv; ++i) {
+        const fk_dim *x = i < nd ? d + i : v + (i - nd);
+        long long st = pick ? x->os : x->is;
+        if (st < 0) return 0;
+        if (x->n > 1) span += (x->n - 1) * st;
+    }
+    return (size_t)(span + 1) * esz;
+}
+
+static void tune_rot_begin(const fk_problem *pb, tune_rot *r)
+{
+    size_t ie, oe, isz, osz, foot;
+    int i, n;
+    memset(r, 0, sizeof(*r));
+    r->n = 1;
+    r->buf[0] = pb->in;
+    r->buf[1] = pb->out;
+    if (pb->split != FK_SPLIT_NONE) return;
+    isz = pb->kind == FK_R2C || pb->kind == FK_R2R ? rsz(pb->prec) : csz(pb->prec);
+    osz = pb->kind == FK_C2R || pb->kind == FK_R2R ? rsz(pb->prec) : csz(pb->prec);
+    ie = array_extent(pb->dims, pb->rank, pb->vdims, pb->vrank, 0, isz);
+    oe = array_extent(pb->dims, pb->rank, pb->vdims, pb->vrank, 1, osz);
+    if (!ie || !oe) return;
+    foot = pb->in == pb->out ? (ie > oe ? ie : oe) : ie + oe;
+    /* beyond the cache already, or too small for TUNE_ROT_MAX copies to
+       leave it: timed on the arrays themselves */
+    if (foot >= TUNE_CACHE_BYTES || foot * TUNE_ROT_MAX < 2 * TUNE_CACHE_BYTES) return;
+    n = (int)((2 * TUNE_CACHE_BYTES + foot - 1) / foot);
+    for (i = 1; i < n; ++i) {
+        void *a = 0, *b = 0;
+        if (pb->in == pb->out) {
+            a = fk_malloc(foot);
+            b = a;
+        } else {
+            a = fk_malloc(ie);
+            b = a ? fk_malloc(oe) : 0;
+        }
+        if (!a || !b) {
+            fk_free(a);
+            break;
+        }
+        /* defined contents, so every copy times the same arithmetic */
+        (void)hipMemset(a, 0, pb->in == pb->out ? foot : ie);
+        if (b != a) (void)hipMemset(b, 0, oe);
+        r->own[2 * i] = a;
+        r->own[2 * i + 1] = b != a ? b : 0;
+        r->buf[2 * i] = a;
+        r->buf[2 * i + 1] = b;
+        r->n = i + 1;
+    }
+    (void)hipGetLastError();
+    (void)hipDeviceSynchronize();
+}
+
+static void tune_rot_end(tune_rot *r)
+{
+    int i;
+    for (i = 0; i < 2 * TUNE_ROT_MAX; ++i) fk_free(r->own[i]);
+    memset(r, 0, sizeof(*r));
+}
+
 static int cmp_dbl(const void *a, const void *b)
 {
     double x = *(const double *)a, y = *(const double *)b;
@@ -2416,6 +2497,7 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
     unsigned long long h = problem_hash(pb);
     fk_plan *P, *Q;
     ctx cb, cq;
+    tune_rot rot;
     double best, t0, sp_best = 0, sp_t = 0;
     if (fk_wisdom_lookup(h, &wlevel, wc, &wn) &&
         (wlevel >= level || (pb->flags & FK_WISDOM_ONLY))) {
@@ -2430,6 +2512,9 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
     P = build_plan(pb, st, 0, 0, &cb);
     if (!P || level == 0 || !P->in || !P->out || P->npass == 0) return P;
     t0 = now_s();
+    tune_rot_begin(pb, &rot);
+    P->rot = rot.buf;
+    P->nrot = rot.n;
     best = time_plan(P, &sp_best);
     ncur = P->nchoice;
     memcpy(cur, P->choice, sizeof(int) * ncur);
@@ -2444,6 +2529,8 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
             trial[i] = a;
             Q = build_plan(pb, st, trial, ncur, &cq);
             if (!Q) continue;
+            Q->rot = rot.buf;
+            Q->nrot = rot.n;
             t = time_plan(Q, &sp_t);
             if (clearly_faster(P, &best, sp_best, Q, t, sp_t)) {
                 fk_plan_destroy(P);
@@ -2458,6 +2545,9 @@ static fk_plan *plan_internal(const fk_problem *pb, hipStream_t st)
         }
     }
 done:
+    P->rot = 0;
+    P->nrot = 0;
+    tune_rot_end(&rot);
     P->pcost = best * 1e-3;
     P->tuned = 2;
     fk_wisdom_store(h, level, cur, ncur);
@@ -2761,8 +2851,17 @@ static double fk_plan_time_run(fk_plan *p, int iter)
     for (i = 0; i < FK_MAX_SCRATCH; ++i) buf[FK_BUF_S0 + i] = p->scratch[i];
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = -1;
     if (!rc) rc = (int)hipEventRecord(e0, st);
-    for (it = 0; it < iter && !rc; ++it)
+    for (it = 0; it < iter && !rc; ++it) {
+        if (p->nrot > 1) {
+            /* the rotation continues across calls (the planner lock is
+               held), so no repeat starts on the copy the last one ended on */
+            static unsigned pos;
+            const int k = (int)(pos++ % (unsigned)p->nrot);
+            buf[FK_BUF_IN] = p->rot[2 * k];
+            buf[FK_BUF_OUT] = p->rot[2 * k + 1];
+        }
         for (i = 0; i < p->npass && !rc; ++i) rc = launch_pass(p->pass + i, buf, st, p->prec);
+    }
     if (!rc) rc = (int)hipEventRecord(e1, st);
     if (!rc) rc = (int)hipEventSynchronize(e1);
     if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = -1;

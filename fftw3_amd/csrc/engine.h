@@ -100,6 +100,10 @@ typedef struct fk_plan {
     int choice[32];             /* planner decisions (wisdom payload) */
     int nchoice;
     int tuned;                  /* 0 heuristic, 1 from wisdom, 2 timed now */
+    /* while FFTW_MEASURE times the plan: nrot > 1 copies of its arrays,
+       rot[2i] / rot[2i + 1] the input / output of copy i, used in turn */
+    void *const *rot;
+    int nrot;
 } fk_plan;
 
 /* problem description handed to the planner (strides in elements of the
