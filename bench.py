@@ -22,7 +22,8 @@ round trip.
 
 The same JSON line carries the other legs of the metric, each bounded in
 time: single-size batched c2c (4096 x 1024 FP64, hot and with buffers
-rotated beyond the 256 MB memory-side cache), the uneven-size batch
+rotated beyond the 256 MB memory-side cache; FP32 and FP64 sizes at batches
+of 512 MiB to 1 GiB per array), the uneven-size batch
 (per-call vs per-size vs persistent+hipGraph transforms/s), the serving loop
 (p50/p99 latency and FFTs/s at a stated Poisson rate), the precision study
 (FP64, FP32, FP16x2 matrix-core at 2^24 x 8) and a DCT-II sub-measure.
@@ -215,6 +216,40 @@ def single_size_c2c(fa, L, n=4096, howmany=1024, iters=50, rotate=8):
     p.destroy()
     del pairs
     return rec
+
+
+def single_size_sweep(fa, L, iters=10):
+    """Single-size batched c2c, out of place, FFTW_MEASURE, at batches whose
+    arrays (512 MiB to 1 GiB each) exceed the 256 MB memory-side cache:
+    per-execute time (best of 3 x `iters`), algorithmic bytes 2*N*H*sizeof(C)
+    against 8 TB/s, and the first and last line checked against scipy.fft."""
+    import scipy.fft
+    out = {}
+    t = Timer(L)
+    for prec, n, H in (("float", 256, 262144), ("float", 1024, 65536), ("float", 4096, 16384),
+                       ("double", 512, 131072), ("double", 4096, 16384)):
+        cs = 16 if prec == "double" else 8
+        dt = np.complex128 if prec == "double" else np.complex64
+        tot = n * H
+        a, b = fa.DeviceBuffer(tot * cs), fa.DeviceBuffer(tot * cs)
+        a.fill_uniform(prec, 2 * tot, 1, 7)
+        p = fa.plan_many_dft(prec, [n], H, a.ptr, None, 1, n, b.ptr, None, 1, n, fa.FFTW_FORWARD,
+                             fa.FFTW_MEASURE)
+        p.execute()
+        L.fftw_amd_synchronize()
+        x = a.download(dt, tot).reshape(H, n)[[0, H - 1]]
+        y = b.download(dt, tot).reshape(H, n)[[0, H - 1]]
+        err = rel_l2(y, scipy.fft.fft(x.astype(np.complex128), axis=1))
+        ms = t.best(lambda: p.execute(), 3, inner=iters)
+        by = 2.0 * tot * cs
+        out["%s %d x%d" % ("fp64" if prec == "double" else "fp32", n, H)] = {
+            "ms": round(ms, 5), "GB_s": round(by / (ms * 1e-3) / 1e9, 1),
+            "frac": round(by / (ms * 1e-3) / 1e9 / PEAK_HBM, 4), "rel_l2_vs_scipy_fp64": err,
+            "passes": [s.split(" lines=")[0] for s in p.pass_info()]}
+        p.destroy()
+        del a, b
+    t.close()
+    return out
 
 
 def r2r_dct2(fa, L, shape=(2048, 2048), howmany=16, iters=20):
@@ -846,6 +881,8 @@ def main():
         legs["single_size_c2c_fp64_4096x1024"] = c1 = single_size_c2c(fa, L)
         log("single-size c2c 4096 x 1024 fp64: hot %.4f ms (%.3f), rotated %.4f ms (%.3f), kernel %.4f ms"
             % (c1["ms_hot"], c1["frac_hot"], c1["ms_rotated"], c1["frac_rotated"], c1["ms_kernel_event"]))
+        legs["single_size_c2c_large_batches"] = sw = single_size_sweep(fa, L)
+        log("single-size c2c, large batches: " + ", ".join("%s %.3f" % (k, v["frac"]) for k, v in sw.items()))
         legs["r2r_redft10_fp32_2048x2048x16"] = r2r_dct2(fa, L)
         legs["mixed_batch_c3"] = run_mixed(fa, L)
         legs["serving_c5"] = run_serve(fa, L, args.requests, args.load)
