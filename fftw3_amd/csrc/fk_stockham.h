@@ -432,12 +432,20 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         /* few bins per thread: their twiddles are loaded before the stages,
            so the load latency hides behind the FFT instead of following it */
         constexpr bool PRE = NK <= 20;
-        cx<R> wpre[PRE ? NK : 1];
+        /* plain and cross-row kinds: bin k <= L/2 also gives bin L-k from
+           the same Z[k], Z[L-k] and twiddle (W_2L^(L-k) = -conj W_2L^k):
+           half the LDS reads and twiddle loads of the post-pass */
+        constexpr bool MIR = IO == FK_IO_R2C || IO == FK_IO_R2C_XR;
+        constexpr int HK = L / 2;
+        constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
+        constexpr int NW = MIR ? NK2 : NK;
+        cx<R> wpre[PRE ? NW : 1];
         if constexpr (PRE) {
-            sfor<0, NK>([&](auto ic) {
+            sfor<0, NW>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const int k = u + i * TPL;
-                wpre[i] = wk[k <= L ? k : L];
+                if constexpr (MIR) wpre[i] = wk[k <= HK ? k : HK];
+                else wpre[i] = wk[k <= L ? k : L];
             });
         }
         /* cross-row form: twiddles of the first column stage, W_n0^(n1*p)
@@ -453,11 +461,49 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* cross-row form: the bins stay in registers until every bin of the
            line has been formed, then replace Z in the line's LDS row */
-        cx<R> xs[XR ? NK : 1];
+        cx<R> xs[XR ? (MIR ? 2 * NK2 : NK) : 1];
         constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
         if constexpr (XR) {
             if (u == 0 && real) lbase[lt] = act ? bout : NONE;
         }
+        if constexpr (MIR) {
+        sfor<0, (NK2 + GK - 1) / GK>([&](auto gc) {
+            constexpr int g = decltype(gc)::value;
+            constexpr int cnt = (NK2 - g * GK) < GK ? (NK2 - g * GK) : GK;
+            cx<R> w[cnt];
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if constexpr (PRE) w[i] = wpre[g * GK + i];
+                else if (k <= HK) w[i] = wk[k];
+            });
+            sfor<0, cnt>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + (g * GK + i) * TPL;
+                if (k <= HK) {
+                    const cx<R> zk = lds_at(k);
+                    const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
+                    const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
+                    const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
+                    const R wx = w[i].x, wy = w[i].y;
+                    /* X[k] = E + W O; X[L-k] = conj(E - W O), written as the
+                       per-bin form of bin L-k (E' = conj E, O' = conj O,
+                       W' = -conj W) so both round as that form does */
+                    cx<R> X = mk(er + orr * wx - oi * wy, ei + orr * wy + oi * wx);
+                    cx<R> Y = mk(er + orr * (-wx) - (-oi) * wy, (-ei) + orr * wy + (-oi) * (-wx));
+                    X.x *= sc; X.y *= sc;
+                    Y.x *= sc; Y.y *= sc;
+                    if constexpr (XR) {
+                        xs[2 * (g * GK + i)] = X;
+                        xs[2 * (g * GK + i) + 1] = Y;
+                    } else if (act) {
+                        out[bout + (long long)k * a.os] = X;
+                        if (2 * k != L) out[bout + (long long)(L - k) * a.os] = Y;
+                    }
+                }
+            });
+        });
+        } else {
         sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
@@ -514,14 +560,26 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 }
             });
         });
+        }
         if constexpr (XR) {
             /* every thread of the line has read Z: the bins replace it */
             SY::sync();
-            sfor<0, NK>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int k = u + i * TPL;
-                if (k <= L) lds_at(k) = xs[i];
-            });
+            if constexpr (MIR) {
+                sfor<0, NK2>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const int k = u + i * TPL;
+                    if (k <= HK) {
+                        lds_at(k) = xs[2 * i];
+                        if (2 * k != L) lds_at(L - k) = xs[2 * i + 1];
+                    }
+                });
+            } else {
+                sfor<0, NK>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const int k = u + i * TPL;
+                    if (k <= L) lds_at(k) = xs[i];
+                });
+            }
             __syncthreads();
             /* first stage of the column DFT over the tile's NL rows
                n = n1 + m*n2 (n2 = the line's batch index i0, n1 = i1; the
@@ -572,6 +630,19 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         };
         constexpr int NK = (L + 1 + TPL - 1) / TPL;
         constexpr int GK = 8;
+        /* plain and cross-row kinds: bins X[k], X[L-k] (k <= L/2) are read
+           together and give the swapped Z[k] and Z[L-k] of `pre` below with
+           one twiddle (W_2L^(L-k) = -conj W_2L^k), written to LDS before the
+           stages, which then read Z as it is */
+        constexpr bool MIRC = IO == FK_IO_C2R || IO == FK_IO_C2R_XR;
+        constexpr int HK = L / 2;
+        auto zpair = [&](cx<R> xk, cx<R> xc, cx<R> w, cx<R> &z0, cx<R> &z1) {
+            const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
+            const cx<R> d = mk(xk.x - xc.x, xk.y + xc.y);
+            const cx<R> o = mk(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
+            z0 = mk(e.y + o.x, e.x - o.y);
+            z1 = mk(o.x - e.y, e.x + o.y);
+        };
         if constexpr (IO == FK_IO_C2R_XR) {
             /* last stage of the inverse column DFT over the tile's NL input
                rows (row p = the line's batch index i0, n1 = i1; the planner
@@ -589,24 +660,69 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             });
             __syncthreads();
             const cx<R> *inb = reinterpret_cast<const cx<R> *>(a.in);
-            for (int k = tid; k <= L; k += NT) {
-                cx<R> v[NL];
+            /* bin k of every row of the tile: inverse DFT through the
+               forward one on swapped data */
+            auto coldft = [&](int k, cx<R> *v) {
                 sfor<0, NL>([&](auto pc) {
                     constexpr int pp = decltype(pc)::value;
                     const long long ib = lbase[pp];
                     const cx<R> y = ib != NONE ? inb[ib + (long long)k * a.is] : mk((R)0, (R)0);
-                    /* inverse DFT through the forward one on swapped data */
                     v[pp] = swp(pp == 0 ? y : mul(y, wp[pp]));
                 });
                 Dft<R, NL, 1>::run(v);
+            };
+            for (int k = tid; k <= HK; k += NT) {
+                const cx<R> w = wk[k];
+                cx<R> v[NL], vm[NL];
+                coldft(k, v);
+                if (2 * k != L) coldft(L - k, vm);
                 sfor<0, NL>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    cx<R> z = swp(v[q]);
-                    if (k == 0 || k == L) z.y = 0;
-                    lds[G::idx(q, k)] = z;
+                    cx<R> xk = swp(v[q]);
+                    cx<R> xc = 2 * k == L ? xk : swp(vm[q]);
+                    if (k == 0) {
+                        xk.y = 0;
+                        xc.y = 0;
+                    }
+                    cx<R> z0, z1;
+                    zpair(xk, xc, w, z0, z1);
+                    lds[G::idx(q, k)] = z0;
+                    if (k != 0 && 2 * k != L) lds[G::idx(q, L - k)] = z1;
                 });
             }
             __syncthreads();
+        } else if constexpr (MIRC) {
+            constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
+            sfor<0, (NK2 + GK - 1) / GK>([&](auto gc) {
+                constexpr int g = decltype(gc)::value;
+                constexpr int cnt = (NK2 - g * GK) < GK ? (NK2 - g * GK) : GK;
+                cx<R> xa[cnt], xb[cnt], w[cnt];
+                sfor<0, cnt>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const int k = u + (g * GK + i) * TPL;
+                    if (k <= HK) {
+                        xa[i] = !act ? mk((R)0, (R)0) : in[(long long)k * a.is];
+                        xb[i] = !act ? mk((R)0, (R)0) : in[(long long)(L - k) * a.is];
+                        w[i] = wk[k];
+                    }
+                });
+                sfor<0, cnt>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    const int k = u + (g * GK + i) * TPL;
+                    if (k <= HK) {
+                        cx<R> xk = xa[i], xc = xb[i];
+                        if (k == 0) {
+                            xk.y = 0;
+                            xc.y = 0;
+                        }
+                        cx<R> z0, z1;
+                        zpair(xk, xc, w[i], z0, z1);
+                        lds_at(k) = z0;
+                        if (k != 0 && 2 * k != L) lds_at(L - k) = z1;
+                    }
+                });
+            });
+            SY::sync();
         } else {
         /* loads in groups of 8 (compile-time count) so that each group is
            in flight at once, then its LDS stores */
@@ -666,7 +782,12 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 outr[(2 * k + 1) * a.os] = v.x;
             }
         };
-        run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, pre, gst);
+        if constexpr (MIRC) {
+            auto zld = [&](auto j) -> cx<R> { return lds_at(j); };
+            run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, zld, gst);
+        } else {
+            run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, pre, gst);
+        }
     } else if constexpr (IO == FK_IO_BLUE) {
         /* Bluestein (dft/bluestein.c:82-128 conventions): a_j = x_j c_j,
            c_j = exp(-i pi j^2/n); conv = IDFT_L(DFT_L(a) * B),
