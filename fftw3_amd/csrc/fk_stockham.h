@@ -424,28 +424,23 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             return mk(inr[(2 * j) * a.is], inr[(2 * j + 1) * a.is]);
         };
         auto lst = [&](auto k, cx<R> v) { lds_at(k) = v; };
-        /* bins k = u + i*TPL in groups of 8 (compile-time count): the
-           group's twiddle loads are in flight together */
-        constexpr int NK = (L + 1 + TPL - 1) / TPL;
+        /* bin k <= L/2 gives bins k and L-k from the same Z[k], Z[L-k] and
+           one twiddle (W_2L^(L-k) = -conj W_2L^k); bins k = u + i*TPL in
+           groups of 8 (compile-time count): the group's twiddle loads are
+           in flight together */
+        constexpr int HK = L / 2;
+        constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
         constexpr int GK = 8;
         constexpr bool XR = IO == FK_IO_R2C_XR;
         /* few bins per thread: their twiddles are loaded before the stages,
            so the load latency hides behind the FFT instead of following it */
-        constexpr bool PRE = NK <= 20;
-        /* plain and cross-row kinds: bin k <= L/2 also gives bin L-k from
-           the same Z[k], Z[L-k] and twiddle (W_2L^(L-k) = -conj W_2L^k):
-           half the LDS reads and twiddle loads of the post-pass */
-        constexpr bool MIR = IO == FK_IO_R2C || IO == FK_IO_R2C_XR;
-        constexpr int HK = L / 2;
-        constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
-        constexpr int NW = MIR ? NK2 : NK;
-        cx<R> wpre[PRE ? NW : 1];
+        constexpr bool PRE = NK2 <= 20;
+        cx<R> wpre[PRE ? NK2 : 1];
         if constexpr (PRE) {
-            sfor<0, NW>([&](auto ic) {
+            sfor<0, NK2>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 const int k = u + i * TPL;
-                if constexpr (MIR) wpre[i] = wk[k <= HK ? k : HK];
-                else wpre[i] = wk[k <= L ? k : L];
+                wpre[i] = wk[k <= HK ? k : HK];
             });
         }
         /* cross-row form: twiddles of the first column stage, W_n0^(n1*p)
@@ -461,12 +456,45 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
         run_stages<R, RL, 0, L, TPL, false, true, SY>(u, tw, lds_at, gld, lst);
         /* cross-row form: the bins stay in registers until every bin of the
            line has been formed, then replace Z in the line's LDS row */
-        cx<R> xs[XR ? (MIR ? 2 * NK2 : NK) : 1];
+        cx<R> xs[XR ? 2 * NK2 : 1];
         constexpr long long NONE = -0x7fffffffffffffffLL - 1; /* INT64_MIN */
         if constexpr (XR) {
             if (u == 0 && real) lbase[lt] = act ? bout : NONE;
         }
-        if constexpr (MIR) {
+        /* bin k's outputs: the complex bin, or the real outputs of the
+           fused real-to-real kind */
+        auto emit = [&](int k, cx<R> X) {
+            if (act && rm == FK_RM_NONE) {
+                out[bout + (long long)k * a.os] = X;
+            } else if (act && rm == FK_RM_E00) {
+                outr[(long long)k * a.os] = X.x;             /* DCT-I: Re X_k */
+            } else if (act && rm == FK_RM_O00) {
+                if (k >= 1 && k < L) outr[(long long)(k - 1) * a.os] = -X.y; /* DST-I */
+            } else if (act) {
+                /* bin k gives two real outputs, k and n-k:
+                   R2HC r_k / i_k; DHT Re -+ Im; REDFT10
+                   c = exp(-i pi k / 2n) X: 2 Re c, -2 Im c */
+                R lo, hi;
+                if (rm == FK_RM_HC) {
+                    lo = X.x;
+                    hi = X.y;
+                } else if (rm == FK_RM_DHT) {
+                    lo = X.x - X.y;
+                    hi = X.x + X.y;
+                } else {
+                    const cx<R> c = mul(X, rt[2 * k]);
+                    lo = (R)2 * c.x;
+                    hi = (R)-2 * c.y;
+                }
+                long long ka = k, kb = N2 - k;
+                if (rm == FK_RM_DST2) {
+                    ka = N2 - 1 - k;
+                    kb = k - 1;
+                }
+                outr[ka * a.os] = lo;
+                if (k != 0 && k != L) outr[kb * a.os] = hi;
+            }
+        };
         sfor<0, (NK2 + GK - 1) / GK>([&](auto gc) {
             constexpr int g = decltype(gc)::value;
             constexpr int cnt = (NK2 - g * GK) < GK ? (NK2 - g * GK) : GK;
@@ -484,11 +512,12 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     const cx<R> zk = lds_at(k);
                     const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
                     const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
+                    /* O = (Z[k] - conj Z[L-k]) / (2i) */
                     const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
                     const R wx = w[i].x, wy = w[i].y;
                     /* X[k] = E + W O; X[L-k] = conj(E - W O), written as the
-                       per-bin form of bin L-k (E' = conj E, O' = conj O,
-                       W' = -conj W) so both round as that form does */
+                       form of bin L-k on its own (E' = conj E, O' = conj O,
+                       W' = -conj W) so that both round as that form does */
                     cx<R> X = mk(er + orr * wx - oi * wy, ei + orr * wy + oi * wx);
                     cx<R> Y = mk(er + orr * (-wx) - (-oi) * wy, (-ei) + orr * wy + (-oi) * (-wx));
                     X.x *= sc; X.y *= sc;
@@ -496,90 +525,24 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     if constexpr (XR) {
                         xs[2 * (g * GK + i)] = X;
                         xs[2 * (g * GK + i) + 1] = Y;
-                    } else if (act) {
-                        out[bout + (long long)k * a.os] = X;
-                        if (2 * k != L) out[bout + (long long)(L - k) * a.os] = Y;
+                    } else {
+                        emit(k, X);
+                        if (2 * k != L) emit(L - k, Y);
                     }
                 }
             });
         });
-        } else {
-        sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
-            constexpr int g = decltype(gc)::value;
-            constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
-            cx<R> w[cnt];
-            sfor<0, cnt>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int k = u + (g * GK + i) * TPL;
-                if constexpr (PRE) w[i] = wpre[g * GK + i];
-                else if (k <= L) w[i] = wk[k];
-            });
-            sfor<0, cnt>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int k = u + (g * GK + i) * TPL;
-                if (k <= L) {
-                    const cx<R> zk = lds_at(k == L ? 0 : k);
-                    const cx<R> zc = lds_at(k == 0 ? 0 : L - k);
-                    const R er = (R)0.5 * (zk.x + zc.x), ei = (R)0.5 * (zk.y - zc.y);
-                    /* O = (Z[k] - conj Z[L-k]) / (2i) */
-                    const R orr = (R)0.5 * (zk.y + zc.y), oi = (R)-0.5 * (zk.x - zc.x);
-                    cx<R> X = mk(er + orr * w[i].x - oi * w[i].y, ei + orr * w[i].y + oi * w[i].x);
-                    X.x *= sc; X.y *= sc;
-                    if constexpr (XR) {
-                        xs[g * GK + i] = X;
-                    } else if (act && rm == FK_RM_NONE) {
-                        out[bout + (long long)k * a.os] = X;
-                    } else if (act && rm == FK_RM_E00) {
-                        outr[(long long)k * a.os] = X.x;             /* DCT-I: Re X_k */
-                    } else if (act && rm == FK_RM_O00) {
-                        if (k >= 1 && k < L) outr[(long long)(k - 1) * a.os] = -X.y; /* DST-I */
-                    } else if (act) {
-                        /* bin k gives two real outputs, k and n-k:
-                           R2HC r_k / i_k; DHT Re -+ Im; REDFT10
-                           c = exp(-i pi k / 2n) X: 2 Re c, -2 Im c */
-                        R lo, hi;
-                        if (rm == FK_RM_HC) {
-                            lo = X.x;
-                            hi = X.y;
-                        } else if (rm == FK_RM_DHT) {
-                            lo = X.x - X.y;
-                            hi = X.x + X.y;
-                        } else {
-                            const cx<R> c = mul(X, rt[2 * k]);
-                            lo = (R)2 * c.x;
-                            hi = (R)-2 * c.y;
-                        }
-                        long long ka = k, kb = N2 - k;
-                        if (rm == FK_RM_DST2) {
-                            ka = N2 - 1 - k;
-                            kb = k - 1;
-                        }
-                        outr[ka * a.os] = lo;
-                        if (k != 0 && k != L) outr[kb * a.os] = hi;
-                    }
-                }
-            });
-        });
-        }
         if constexpr (XR) {
             /* every thread of the line has read Z: the bins replace it */
             SY::sync();
-            if constexpr (MIR) {
-                sfor<0, NK2>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    const int k = u + i * TPL;
-                    if (k <= HK) {
-                        lds_at(k) = xs[2 * i];
-                        if (2 * k != L) lds_at(L - k) = xs[2 * i + 1];
-                    }
-                });
-            } else {
-                sfor<0, NK>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    const int k = u + i * TPL;
-                    if (k <= L) lds_at(k) = xs[i];
-                });
-            }
+            sfor<0, NK2>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const int k = u + i * TPL;
+                if (k <= HK) {
+                    lds_at(k) = xs[2 * i];
+                    if (2 * k != L) lds_at(L - k) = xs[2 * i + 1];
+                }
+            });
             __syncthreads();
             /* first stage of the column DFT over the tile's NL rows
                n = n1 + m*n2 (n2 = the line's batch index i0, n1 = i1; the
@@ -628,14 +591,16 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
             const cx<R> w = rt[2 * k];
             return mk(w.x * xk - w.y * xn, -(w.x * xn + w.y * xk));
         };
-        constexpr int NK = (L + 1 + TPL - 1) / TPL;
-        constexpr int GK = 8;
-        /* plain and cross-row kinds: bins X[k], X[L-k] (k <= L/2) are read
-           together and give the swapped Z[k] and Z[L-k] of `pre` below with
-           one twiddle (W_2L^(L-k) = -conj W_2L^k), written to LDS before the
-           stages, which then read Z as it is */
-        constexpr bool MIRC = IO == FK_IO_C2R || IO == FK_IO_C2R_XR;
+        /* bins X[k], X[L-k] (k <= L/2) are read together and give Z[k] and
+           Z[L-k] with one twiddle (W_2L^(L-k) = -conj W_2L^k); they are
+           written to LDS before the stages, which read Z as it is */
         constexpr int HK = L / 2;
+        constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
+        constexpr int GK = 8;
+        /* swapped Z (the backward DFT runs as the forward one on swapped
+           data): e = X[k] + conj X[L-k], d = X[k] - conj X[L-k],
+           o = d conj(W_2L^k); Z[k] = e + i o and, by the symmetry above,
+           Z[L-k] = conj e + i conj o */
         auto zpair = [&](cx<R> xk, cx<R> xc, cx<R> w, cx<R> &z0, cx<R> &z1) {
             const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
             const cx<R> d = mk(xk.x - xc.x, xk.y + xc.y);
@@ -691,8 +656,9 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 });
             }
             __syncthreads();
-        } else if constexpr (MIRC) {
-            constexpr int NK2 = (HK + 1 + TPL - 1) / TPL;
+        } else {
+            /* loads in groups of 8 (compile-time count) so that each group
+               is in flight at once, then its LDS stores */
             sfor<0, (NK2 + GK - 1) / GK>([&](auto gc) {
                 constexpr int g = decltype(gc)::value;
                 constexpr int cnt = (NK2 - g * GK) < GK ? (NK2 - g * GK) : GK;
@@ -701,8 +667,10 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                     constexpr int i = decltype(ic)::value;
                     const int k = u + (g * GK + i) * TPL;
                     if (k <= HK) {
-                        xa[i] = !act ? mk((R)0, (R)0) : in[(long long)k * a.is];
-                        xb[i] = !act ? mk((R)0, (R)0) : in[(long long)(L - k) * a.is];
+                        xa[i] = !act ? mk((R)0, (R)0)
+                                : rm == FK_RM_NONE ? in[(long long)k * a.is] : rbin(k);
+                        xb[i] = !act ? mk((R)0, (R)0)
+                                : rm == FK_RM_NONE ? in[(long long)(L - k) * a.is] : rbin(L - k);
                         w[i] = wk[k];
                     }
                 });
@@ -723,41 +691,7 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 });
             });
             SY::sync();
-        } else {
-        /* loads in groups of 8 (compile-time count) so that each group is
-           in flight at once, then its LDS stores */
-        sfor<0, (NK + GK - 1) / GK>([&](auto gc) {
-            constexpr int g = decltype(gc)::value;
-            constexpr int cnt = (NK - g * GK) < GK ? (NK - g * GK) : GK;
-            cx<R> xin[cnt];
-            sfor<0, cnt>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int k = u + (g * GK + i) * TPL;
-                if (k <= L)
-                    xin[i] = !act ? mk((R)0, (R)0)
-                             : rm == FK_RM_NONE ? in[(long long)k * a.is] : rbin(k);
-            });
-            sfor<0, cnt>([&](auto ic) {
-                constexpr int i = decltype(ic)::value;
-                const int k = u + (g * GK + i) * TPL;
-                if (k <= L) {
-                    cx<R> v = xin[i];
-                    if (k == 0 || k == L) v.y = 0;
-                    lds_at(k) = v;
-                }
-            });
-        });
-        SY::sync();
         }
-        auto pre = [&](int j) -> cx<R> {
-            const cx<R> xk = lds_at(j), xc = lds_at(L - j);
-            const cx<R> e = mk(xk.x + xc.x, xk.y - xc.y);
-            const cx<R> d = mk(xk.x - xc.x, xk.y + xc.y);
-            const cx<R> w = wk[j];            /* W_2L^j; use its conjugate */
-            const cx<R> o = mk(d.x * w.x + d.y * w.y, d.y * w.x - d.x * w.y);
-            /* backward DFT through swapped forward: feed swap(Z) */
-            return mk(e.y + o.x, e.x - o.y);
-        };
         auto gst = [&](int k, cx<R> v) {
             /* v = swap(z_k) */
             v.x *= sc; v.y *= sc;
@@ -782,12 +716,8 @@ __device__ __forceinline__ void stockham_body(const fk_args &a, long long tile,
                 outr[(2 * k + 1) * a.os] = v.x;
             }
         };
-        if constexpr (MIRC) {
-            auto zld = [&](auto j) -> cx<R> { return lds_at(j); };
-            run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, zld, gst);
-        } else {
-            run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, pre, gst);
-        }
+        auto zld = [&](auto j) -> cx<R> { return lds_at(j); };
+        run_stages<R, RL, 0, L, TPL, true, false, SY>(u, tw, lds_at, zld, gst);
     } else if constexpr (IO == FK_IO_BLUE) {
         /* Bluestein (dft/bluestein.c:82-128 conventions): a_j = x_j c_j,
            c_j = exp(-i pi j^2/n); conv = IDFT_L(DFT_L(a) * B),
