@@ -480,8 +480,6 @@ def force_kernel(i):
     """Plans created from now on use catalog entry i where the planner picks
     a kernel of its precision, length, mapping and IO kind (-1: normal)."""
     rc = lib().fftw_amd_force_kernel(i)
-    if rc == -2:
-        raise LookupError("catalog entry %d is never used by the planner" % i)
     if rc:
         raise ValueError("catalog index %d out of range" % i)
 

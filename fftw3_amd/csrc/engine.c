@@ -263,25 +263,13 @@ fail:
 /* ------------------------------------------------------------------ */
 /* kernel catalog queries                                              */
 
-/* Catalog entries the planner never picks: the FP64 column-mapped c2r
-   kernels with a fused real-to-real kind and one line per workgroup
-   (L = 5120, 6144, 8192).  An execute through the 8192-point one aborted
-   on the device and the cause is not established; strided HC2R / REDFT01 /
-   RODFT01 lines of those lengths take the row-mapped kernel instead. */
-static int kernel_disabled(const fk_kernel *k)
-{
-    return k->prec == 0 && k->mode == FK_COL && k->io == FK_IO_C2R_RR && k->nl == 1;
-}
-
 static const fk_kernel *find_kernel(int prec, long long L, int mode, int io)
 {
     int i;
     if (L > (1 << 20)) return 0;
     for (i = 0; i < g_nreg; ++i) {
         const fk_kernel *k = g_reg + i;
-        if (k->prec == prec && k->L == L && k->mode == mode && k->io == io &&
-            !kernel_disabled(k))
-            return k;
+        if (k->prec == prec && k->L == L && k->mode == mode && k->io == io) return k;
     }
     return 0;
 }
@@ -296,8 +284,7 @@ static const fk_kernel *find_kernel_r16(int prec, long long L, int mode, int io)
     for (i = 0; i < g_nreg; ++i) {
         const fk_kernel *k = g_reg + i;
         int all16 = k->nrad > 0;
-        if (k->prec != prec || k->L != L || k->mode != mode || k->io != io || kernel_disabled(k))
-            continue;
+        if (k->prec != prec || k->L != L || k->mode != mode || k->io != io) continue;
         for (j = 0; j < k->nrad; ++j) all16 &= k->rad[j] == 16;
         if (all16) return k;
     }
@@ -374,7 +361,7 @@ static int find_variants(int prec, long long L, int mode, int io, const fk_kerne
                 break;
             }
         }
-        if (!hit || kernel_disabled(hit)) break;
+        if (!hit) break;
         out[n++] = hit;
     }
     return n;
@@ -417,7 +404,6 @@ int fk_force_kernel(int i)
 {
     const fk_kernel *k = i < 0 ? 0 : fk_catalog_entry(i);
     if (i >= 0 && !k) return -1;
-    if (k && kernel_disabled(k)) return -2;
     pthread_mutex_lock(&g_lock);
     g_force = k;
     pthread_mutex_unlock(&g_lock);

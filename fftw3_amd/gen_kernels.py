@@ -221,7 +221,12 @@ def catalog():
                         # serves r2r dims whose lines are strided
                         entries.append((prec, L, rads, "ROW", io + "_RR", g))
                         gc = geometry(prec, L, rads, "COL", io)
-                        if gc:
+                        # not for FP64 c2r at one line per workgroup
+                        # (L = 5120, 6144, 8192): an execute through the
+                        # 8192-point one aborted on the device, cause not
+                        # established; strided lines of those lengths take
+                        # the row-mapped kernel
+                        if gc and not (prec == "double" and io == "C2R" and gc["nl"] == 1):
                             entries.append((prec, L, rads, "COL", io + "_RR", gc))
             # half-length r2c / c2r fused with one radix-8 stage of the next
             # dimension's DFT across the 8 rows of a tile (2-D and higher

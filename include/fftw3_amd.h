@@ -109,9 +109,7 @@ int fftw_amd_catalog_entry(int i, int *info);
    the planner would pick a kernel of the same precision, length, mapping
    and IO kind (whichever variant it would have chosen); -1 restores normal
    planning.  Lets every catalog entry, including the variants only the
-   autotuner picks, be run and checked.  Returns 0, -1 for a bad index, or
-   -2 for an entry the planner never uses (the FP64 column-mapped c2r
-   kernels with a fused real-to-real kind and one line per workgroup). */
+   autotuner picks, be run and checked.  Returns 0, or -1 for a bad index. */
 int fftw_amd_force_kernel(int i);
 
 /* Split-precision matrix-core variant (FP32 plans only): every pass of

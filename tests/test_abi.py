@@ -72,3 +72,16 @@ def test_wisdom_string_roundtrip_contract():
     assert L.fftw_import_wisdom_from_string(s) == 1
     assert L.fftw_import_wisdom_from_string(b"(fftw-3.3.10 fftw_wisdom #x0 #x0 #x0 #x0)") == 0
     assert L.fftw_import_wisdom_from_string(b"garbage") == 0
+
+
+def test_catalog_leaves_out_fp64_column_c2r_r2r_single_line():
+    """The FP64 column-mapped c2r kernels with a fused real-to-real kind and
+    one line per workgroup (L = 5120, 6144, 8192) are not built: an execute
+    through the 8192-point one aborted on the device, cause not established.
+    Strided lines of those lengths take the row-mapped kernel."""
+    import fftw3_amd as fa
+    cat = fa.catalog()
+    assert len(cat) == fa.lib().fftw_amd_catalog_size()
+    col = [e for e in cat if e["prec"] == "double" and e["mode"] == "col" and e["io"] == "c2r-r2r"]
+    assert col and all(e["nl"] > 1 for e in col)
+    assert not [e for e in col if e["L"] in (5120, 6144, 8192)]

@@ -236,21 +236,8 @@ def _id(e):
                                    e["mode"], e["io"], e["variant"])
 
 
-def disabled(e):
-    """FP64 column-mapped c2r kernels with a fused real-to-real kind and one
-    line per workgroup: the planner never uses them (an execute through the
-    8192-point one aborted on the device, cause not established)."""
-    return e["prec"] == "double" and e["mode"] == "col" and e["io"] == "c2r-r2r" and e["nl"] == 1
-
-
 @pytest.mark.parametrize("e", CAT, ids=[_id(e) for e in CAT])
 def test_catalog_entry(forced, e):
-    if disabled(e):
-        with pytest.raises(LookupError):
-            fa.force_kernel(e["index"])
-        pytest.skip("planner never uses this entry: an execute through the FP64 column c2r "
-                    "real-to-real kernel of 8192 points aborted on the device, cause not "
-                    "established")
     fa.force_kernel(e["index"])
     try:
         for label, thunk in cases(e):
