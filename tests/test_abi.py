@@ -85,3 +85,42 @@ def test_catalog_leaves_out_fp64_column_c2r_r2r_single_line():
     col = [e for e in cat if e["prec"] == "double" and e["mode"] == "col" and e["io"] == "c2r-r2r"]
     assert col and all(e["nl"] > 1 for e in col)
     assert not [e for e in col if e["L"] in (5120, 6144, 8192)]
+
+
+def kernel_metadata(obj, tmp):
+    """(name, private segment bytes, VGPRs) of every gfx950 kernel in one
+    built HIP object: its offload bundle, unbundled, then the code object's
+    metadata notes."""
+    llvm = "/opt/rocm/llvm/bin"
+    fat, co = os.path.join(tmp, "fat.bin"), os.path.join(tmp, "k.co")
+    subprocess.run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin", obj, fat], check=True)
+    subprocess.run([os.path.join(llvm, "clang-offload-bundler"), "--unbundle", "--type=o",
+                    "--input=" + fat, "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
+                    "--output=" + co], check=True)
+    notes = subprocess.run([os.path.join(llvm, "llvm-readelf"), "--notes", co],
+                           capture_output=True, text=True, check=True).stdout
+    out = []
+    for blk in notes.split("  - .agpr_count:")[1:]:
+        name = re.search(r"\.name:\s+(\S+)", blk).group(1)
+        priv = int(re.search(r"\.private_segment_fixed_size:\s+(\d+)", blk).group(1))
+        vgpr = int(re.search(r"\.vgpr_count:\s+(\d+)", blk).group(1))
+        out.append((name, priv, vgpr))
+    return out
+
+
+def test_catalog_kernels_do_not_spill(tmp_path):
+    """Every instantiated Stockham kernel keeps its registers: no private
+    (scratch) segment, which would mean spills to memory."""
+    gen = os.path.join(ROOT, "fftw3_amd", "_build", "gen")
+    objs = sorted(f for f in os.listdir(gen) if f.startswith("fk_inst_")) if os.path.isdir(gen) else []
+    if not objs or not os.path.exists("/opt/rocm/llvm/bin/llvm-readelf"):
+        pytest.skip("built kernel objects or ROCm's LLVM tools are not here")
+    import fftw3_amd as fa
+    seen, spills = 0, []
+    for f in objs:
+        for name, priv, vgpr in kernel_metadata(os.path.join(gen, f), str(tmp_path)):
+            seen += 1
+            if priv:
+                spills.append((name, priv, vgpr))
+    assert seen == len(fa.catalog())
+    assert not spills, spills[:10]
